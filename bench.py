@@ -1,0 +1,175 @@
+"""Benchmark: 3D Rayleigh-Taylor, compressible Navier-Stokes, 256^3, FP64.
+
+One "step" is one SSPRK3 time step (three fused stages plus the CFL step)
+over the 256^3 grid: MUSCL + van Leer, HLL, centred viscous flux
+(mu = 1e-4), gravity g = -0.1 in x3, periodic x1/x2 and reflecting x3 walls
+(momentum_x3 reflecting, the other fields Neumann).  The state is resident
+in HBM before the timed region; dt stays on the device.
+
+Prints one JSON line (metric Mcell-updates/s) with the roofline of the
+dominant kernel (the fused stage kernel, HIP events around each launch inside
+the timed region) and the CPU baseline (the NumPy oracle on a bounded sample
+of the same scheme, timed on this host).
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "multiphysics-fvm_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+# Compulsory HBM traffic of one SSPRK3 step per cell (float64, 5 fields):
+# stage 1 reads U^n and writes U1 (80 B); stages 2 and 3 read U^s and U^n and
+# write (120 B each).
+BYTES_PER_CELL_STEP = 320.0
+STAGES = 3
+
+
+def rt_config(n):
+    return {"x1_min": 0.0, "x1_max": 1.0, "nx1": n, "x2_min": 0.0, "x2_max": 1.0, "nx2": n,
+            "x3_min": 0.0, "x3_max": 1.0, "nx3": n}
+
+
+def rt_fields(n, seed=1):
+    """Heavy fluid (rho=2) over light (rho=1), hydrostatic p with p=2.5 at z=0.5,
+    vz = 0.01 (1+cos 2pi x)(1+cos 2pi y)/4 + U(-1e-3, 1e-3)."""
+    rng = np.random.default_rng(seed)
+    c = (np.arange(n) + 0.5) / n
+    X, Y, Z = np.meshgrid(c, c, c, indexing="ij")
+    heavy = Z > 0.5
+    rho = np.where(heavy, 2.0, 1.0)
+    g = 0.1
+    p = np.where(heavy, 2.5 - g * 2.0 * (Z - 0.5), 2.5 + g * 1.0 * (0.5 - Z))
+    vz = 0.01 * (1 + np.cos(2 * np.pi * X)) * (1 + np.cos(2 * np.pi * Y)) / 4 \
+        + rng.uniform(-1e-3, 1e-3, (n, n, n))
+    E = p / 0.4 + 0.5 * rho * vz * vz
+    z = np.zeros_like(rho)
+    return [rho, z, z, rho * vz, E]
+
+
+PARAMS = {"heat_capacity_ratio": 1.4, "dynamic_viscosity": 1e-4,
+          "gravitational_acceleration": [0.0, 0.0, -0.1]}
+CFL = 0.4
+
+
+def bc_rows():
+    # [left,right,bottom,top,front,back]: periodic x1/x2; x3 walls
+    out = []
+    for v in range(5):
+        out.append([1, 1, 1, 1, 2, 2] if v == 3 else [1, 1, 1, 1, 0, 0])
+    return out
+
+
+def cpu_baseline(n=80, steps=2):
+    """The NumPy oracle on a bounded sample (same scheme, n^3, `steps` steps)."""
+    from oracle import grid as og
+    from oracle import navier_stokes as ons
+    from oracle import numerics as on
+    from oracle import solver as osv
+    cfg = rt_config(n)
+    U = ons.allocate_u(cfg)
+    for k, f in zip(ons.VARS, rt_fields(n)):
+        U[k][1:-1, 1:-1, 1:-1] = f
+    rule = {0: og.neumann, 1: og.periodic, 2: og.reflecting}
+    bcs = {k: {s: rule[c] for s, c in zip(og.SIDES, row)} for k, row in zip(ons.VARS, bc_rows())}
+    og.apply_bcs(U, bcs)
+    g = og.build_grid("cartesian", cfg)
+    schemes = {"riemann_solver": on.hll, "reconstructor": on.muscl, "slope_limiter": on.van_leer,
+               "diffusive_solver": on.centered}
+    t0 = time.perf_counter()
+    osv.advance(ons, U, g, PARAMS, schemes, on.rk3_ssp, bcs, CFL, steps)
+    wall = time.perf_counter() - t0
+    return {"value": n ** 3 * steps / wall / 1e6, "unit": "Mcell-updates/s", "cores": 1,
+            "kind": "port",
+            "sample": f"NumPy oracle, RT {n}^3, {steps} SSPRK3 steps (same scheme), "
+                      f"{wall:.1f} s wall on 1 host core"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tile", type=str, default="")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
+        sys.exit(2)
+
+    import torch
+
+    from mfvm import _lib
+    from mfvm.grid import boundary_conditions as bc
+    from mfvm.grid.geometry import build_grid
+    from mfvm.numerics import diffusive_solvers, reconstructors, riemann_solvers, slope_limiters
+    from mfvm.physics import navier_stokes as ns
+
+    n = args.n
+    cfg = rt_config(n)
+    grid = build_grid("cartesian", cfg)
+    U = ns.allocate_u(cfg)
+    for k, f in zip(ns.VARS, rt_fields(n)):
+        U[k][1:-1, 1:-1, 1:-1] = torch.from_numpy(f).cuda()
+    rule = {0: bc.neumann, 1: bc.periodic, 2: bc.reflecting}
+    bcs = {k: {s: rule[c] for s, c in zip(bc.SIDES, row)} for k, row in zip(ns.VARS, bc_rows())}
+    schemes = {"riemann_solver": riemann_solvers.hll, "reconstructor": reconstructors.muscl,
+               "slope_limiter": slope_limiters.van_leer,
+               "diffusive_solver": diffusive_solvers.centered}
+    tile = tuple(int(x) for x in args.tile.split(",")) if args.tile else None
+
+    def run(k):
+        return ns.advance(U, grid, PARAMS, schemes, bcs, _lib.INT_SSPRK3, CFL, k, tile=tile)
+
+    run(args.warmup)
+    torch.cuda.synchronize()
+    _lib.timing(1)
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    ev0.record()
+    dts = run(args.steps)
+    ev1.record()
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1)
+    kms, kcnt = _lib.timing(2)
+    _lib.timing(0)
+    assert bool(torch.isfinite(dts).all()), "non-finite dt in the timed run"
+
+    cells = n ** 3
+    value = cells * args.steps / (ms / 1e3) / 1e6
+    stage_ms = kms[_lib.TK_STAGE] / max(kcnt[_lib.TK_STAGE], 1)
+    bytes_per_launch = BYTES_PER_CELL_STEP / STAGES * cells
+    achieved = bytes_per_launch / (stage_ms / 1e3) / 1e9
+    out = {
+        "metric": "Mcell-updates/s", "value": round(value, 2), "unit": "Mcell-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms / args.steps, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"3D Rayleigh-Taylor compressible NS {n}^3, MUSCL+van Leer, HLL, "
+                               "SSPRK3, mu=1e-4, g=-0.1 x3, periodic x1/x2, reflecting x3",
+                   "cells": cells, "integrator": "SSPRK3", "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "ns_stage_kernel", "avg_launch_ms": round(stage_ms, 4),
+                     "bytes_per_launch": bytes_per_launch},
+        "step_hbm_frac": round(value * 1e6 * BYTES_PER_CELL_STEP / (HBM_PEAK_GBS * 1e9), 4),
+        "kernel_ms_per_step": {name: round(kms[i] / args.steps, 4) for i, name in
+                               enumerate(("stage", "closure", "bc", "other"))},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

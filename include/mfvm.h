@@ -1,0 +1,220 @@
+/*
+ * mfvm.h -- C-ABI of the MI355X (gfx950) finite-volume right-hand-side and
+ * time-step library (libmfvm.so).
+ *
+ * The reference (multiphysics-fvm) is Python/NumPy; its "plugin API" is the
+ * setup dict of callables consumed by src/solver.py.  Each entry point below
+ * replaces one of those callables (cited per function) and is bound from
+ * Python with ctypes by the mfvm package.  Plain pointers and sizes only.
+ *
+ * Conventions
+ *  - Memory: the caller owns every buffer (device memory).  The library
+ *    allocates nothing; scratch comes in through the arguments.
+ *  - Layout: a ghosted state of nvar fields is one SoA buffer
+ *    [nvar][nx3+2][nx2+2][nx1+2] (x1 fastest, one ghost layer per side).
+ *    Interior-only buffers (right-hand sides, accumulators) are
+ *    [nvar][nx3][nx2][nx1].  dtype 0 = float64, 1 = float32.
+ *  - Streams: every call is enqueued on the given hipStream_t (NULL = the
+ *    null stream) and returns without synchronising, except where noted.
+ *  - Errors: 0 on success; a negative MFVM_E_* code otherwise, with a
+ *    message in mfvm_last_error() (thread-local).
+ */
+#ifndef MFVM_H
+#define MFVM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ---------------------------------------------------------- */
+#define MFVM_OK 0
+#define MFVM_E_ARG (-1)     /* invalid argument (the Python layer raises ValueError) */
+#define MFVM_E_HIP (-2)     /* HIP runtime error (RuntimeError) */
+#define MFVM_E_UNSUP (-3)   /* unsupported scheme combination (NotImplementedError) */
+
+/* ---- enums ------------------------------------------------------------------ */
+#define MFVM_CART 0
+#define MFVM_CYL 1
+#define MFVM_SPH 2
+
+#define MFVM_LIM_NONE 0     /* reconstructors.piecewise_constant */
+#define MFVM_LIM_MINMOD 1   /* reconstructors.muscl + slope_limiters.minmod */
+#define MFVM_LIM_MC 2       /* ... + monotonized_central */
+#define MFVM_LIM_VANLEER 3  /* ... + van_leer */
+
+#define MFVM_RS_HLL 0       /* riemann_solvers.hll */
+#define MFVM_RS_RUSANOV 1   /* riemann_solvers.rusanov */
+
+#define MFVM_BC_NEUMANN 0   /* boundary_conditions.neumann */
+#define MFVM_BC_PERIODIC 1  /* boundary_conditions.periodic */
+#define MFVM_BC_REFLECTING 2 /* boundary_conditions.reflecting */
+
+#define MFVM_INT_EULER 0    /* time_integrators.euler_explicit */
+#define MFVM_INT_RK3 1      /* time_integrators.rk3 */
+#define MFVM_INT_SSPRK3 2   /* time_integrators.rk3_ssp */
+#define MFVM_INT_RK4 3      /* time_integrators.rk4 */
+
+#define MFVM_F64 0
+#define MFVM_F32 1
+
+/* Metric slots.  Metric m at cell/face (i,j,k) is (a[i]*b[j])*c[k] with
+ * a = fac + off[m][0], b = fac + off[m][1], c = fac + off[m][2] (0-based
+ * interior or face indices).  This reproduces geometry.build_grid
+ * (src/grid/geometry.py:80-171) value for value. */
+#define MFVM_MET_V 0
+#define MFVM_MET_S1 1
+#define MFVM_MET_S2 2
+#define MFVM_MET_S3 3
+#define MFVM_MET_L1 4
+#define MFVM_MET_L2 5
+#define MFVM_MET_L3 6
+#define MFVM_MET_H1 7
+#define MFVM_MET_H2 8
+#define MFVM_MET_H3 9
+#define MFVM_MET_D1H 10   /* d/dx1 ln(h1 h2 h3) */
+#define MFVM_MET_D1H2 11
+#define MFVM_MET_D1H3 12
+#define MFVM_MET_D2H 13
+#define MFVM_MET_D2H1 14
+#define MFVM_MET_D2H3 15
+#define MFVM_MET_D3H 16
+#define MFVM_MET_D3H1 17
+#define MFVM_MET_D3H2 18
+#define MFVM_NMET 19
+
+/* ---- descriptors ------------------------------------------------------------ */
+typedef struct mfvm_grid {
+  int32_t n[3];                 /* interior cells nx1, nx2, nx3 */
+  int32_t coord;                /* MFVM_CART / MFVM_CYL / MFVM_SPH */
+  const double* fac;            /* device: packed 1D metric factors */
+  int32_t off[MFVM_NMET][3];
+  int32_t lpow2[3];             /* 1 if L_d is the same power of two in every cell */
+  int32_t vpow2;                /* 1 if V is the same power of two in every cell */
+  double lval[3];               /* that L_d (when lpow2[d]) */
+  double vval;                  /* that V (when vpow2) */
+  double smax;                  /* max over cells of 1/L1^2 + 1/L2^2 + 1/L3^2 */
+} mfvm_grid;
+
+typedef struct mfvm_ns_params {  /* navier_stokes.py: params + space_schemes */
+  int32_t dtype;
+  int32_t limiter;              /* MFVM_LIM_* */
+  int32_t riemann;              /* MFVM_RS_* */
+  int32_t bc[5][6];             /* [var][left,right,bottom,top,front,back] */
+  double gamma;                 /* heat_capacity_ratio */
+  double mu;                    /* dynamic_viscosity */
+  double g[3];                  /* gravitational_acceleration */
+  int32_t tile[3];              /* stage tile (x1 threads, x2 rows, x3 planes); 0 = default */
+} mfvm_ns_params;
+
+/* Stage update modes (time_integrators.py).  k is the right-hand side. */
+#define MFVM_UPD_NONE 0   /* no state update (compute_rhs) */
+#define MFVM_UPD_EULER 1  /* out = U + dt*k                               (:47) */
+#define MFVM_UPD_BASE 2   /* out = U0 + (c0*dt)*k                   (:91,97,203,209,215) */
+#define MFVM_UPD_SSP 3    /* out = c0*U0 + c1*(U + dt*k)                 (:153,159) */
+#define MFVM_UPD_ACC6 4   /* out = U0 + (dt/6)*(kacc + k)               (:103,221) */
+
+#define MFVM_ACC_NONE 0
+#define MFVM_ACC_SET 1    /* kacc = k */
+#define MFVM_ACC_ADD 2    /* kacc = kacc + acc_coef*k */
+
+typedef struct mfvm_stage {
+  const void* U;                /* stage input (ghosts filled): stencil reads */
+  const void* C;                /* closure of U (mfvm_ns_closure) */
+  const void* U0;               /* base state (pointwise reads) */
+  void* out;                    /* stage output: interior written */
+  void* kacc;                   /* interior accumulator / rhs output */
+  int32_t update;               /* MFVM_UPD_* */
+  int32_t acc;                  /* MFVM_ACC_* */
+  double c0, c1, acc_coef;
+  double dt;                    /* used when dt_dev is NULL */
+  const double* dt_dev;         /* device scalar dt, or NULL */
+  double* partials;             /* non-NULL: reduce CFL rates of out */
+} mfvm_stage;
+
+typedef struct mfvm_ns_work {   /* scratch for whole steps */
+  void* U;                      /* the state (ghosted), advanced in place */
+  void* S1;                     /* ghosted stage states */
+  void* S2;
+  void* C;                      /* ghosted closure, 7 fields; ghost entries must be 0 */
+  void* kacc;                   /* interior, 5 fields */
+  double* partials;             /* >= mfvm_ns_partials_size() doubles */
+  double* dts;                  /* device: dt of step s written to dts[s] */
+} mfvm_ns_work;
+
+/* ---- library ------------------------------------------------------------------ */
+const char* mfvm_last_error(void);
+int32_t mfvm_abi_version(void);
+
+/* Kernel timing with HIP events recorded on the launch stream around every
+ * kernel of the given kinds (measurement support; no reference counterpart).
+ * mode 1: enable and reset; mode 0: disable; mode 2: wait for the recorded
+ * events and return per-kind totals: ms[k] milliseconds and count[k]
+ * launches for k = MFVM_TK_*.  Host-thread local, not for concurrent use. */
+#define MFVM_TK_STAGE 0
+#define MFVM_TK_CLOSURE 1
+#define MFVM_TK_BC 2
+#define MFVM_TK_OTHER 3
+#define MFVM_TK_N 4
+int mfvm_timing(int32_t mode, double* ms, int64_t* count);
+
+/* Ghost fill of nvar fields: face ghosts by bc[v][side], then edge and corner
+ * averages.  Reads the interior of src, writes the ghost shell of dst
+ * (src == dst is the in-place reference behaviour).
+ * Replaces src/solver.py:13-61 (apply_bcs) with
+ * src/grid/boundary_conditions.py:13-91 (neumann, periodic, reflecting). */
+int mfvm_apply_bcs(const mfvm_grid* g, int32_t nvar, const int32_t* bc, int32_t dtype,
+                   const void* src, void* dst, void* stream);
+
+/* Copy the interior of src into the interior of dst (ghosts untouched). */
+int mfvm_copy_interior(const mfvm_grid* g, int32_t nvar, int32_t dtype,
+                       const void* src, void* dst, void* stream);
+
+/* ---- compressible Navier-Stokes (src/physics/navier_stokes.py) -------------- */
+
+/* Closure C = {P, T11, T22, T33, T12, T13, T23} (7 ghosted fields; P on
+ * interior and face ghosts, T on the interior only, ghost T left untouched
+ * and expected to be 0).  Replaces navier_stokes.py:44-156 (compute_closure). */
+int mfvm_ns_closure(const mfvm_grid* g, const mfvm_ns_params* p, const void* U, void* C,
+                    void* stream);
+
+/* Right-hand side into an interior buffer; C is scratch for the closure.
+ * Replaces navier_stokes.py:588-711 (compute_rhs). */
+int mfvm_ns_rhs(const mfvm_grid* g, const mfvm_ns_params* p, const void* U, void* C,
+                void* rhs, void* stream);
+
+/* CFL time step of U into dt_out (device scalar).  partials: scratch of
+ * mfvm_ns_partials_size() doubles.  Replaces navier_stokes.py:512-585
+ * (compute_dt). */
+int mfvm_ns_dt(const mfvm_grid* g, const mfvm_ns_params* p, const void* U, double cfl,
+               double* partials, double* dt_out, void* stream);
+
+/* One fused Runge-Kutta stage: flux divergence of st->U (closure st->C),
+ * sources, and the update of st->update, optionally the CFL partials of the
+ * result.  Replaces one stage block of src/numerics/time_integrators.py
+ * (e.g. rk3_ssp :144-147) including the compute_rhs call. */
+int mfvm_ns_stage(const mfvm_grid* g, const mfvm_ns_params* p, const mfvm_stage* st,
+                  void* stream);
+
+/* One integrator call with a given dt: ghost fill, closure and fused stage
+ * per Runge-Kutta stage, state advanced in place in w->U (w->partials and
+ * w->dts unused).  Replaces time_integrators.py:10-221 (euler_explicit, rk3,
+ * rk3_ssp, rk4) called with navier_stokes.compute_rhs and solver.apply_bcs. */
+int mfvm_ns_step(const mfvm_grid* g, const mfvm_ns_params* p, const mfvm_ns_work* w,
+                 int32_t integrator, double dt, void* stream);
+
+/* nsteps whole steps of the loop body of src/solver.py:137-151 (dt from the
+ * current state, one integrator call), dt kept on the device: dts[s] is the
+ * dt of step s and dts[nsteps] the dt of the following step (nsteps+1
+ * entries). */
+int mfvm_ns_advance(const mfvm_grid* g, const mfvm_ns_params* p, const mfvm_ns_work* w,
+                    int32_t integrator, double cfl, int32_t nsteps, void* stream);
+
+int64_t mfvm_ns_partials_size(const mfvm_grid* g, const mfvm_ns_params* p);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MFVM_H */

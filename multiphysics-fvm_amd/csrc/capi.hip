@@ -1,0 +1,349 @@
+// C-ABI of libmfvm.so: argument checks, error reporting and stage sequencing.
+#include <string>
+#include <vector>
+
+#include "common.h"
+
+namespace mfvm {
+
+// Defined in ns.hip / bc.hip.
+template <class R> hipError_t launch_closure(const mfvm_grid*, const mfvm_ns_params*, const void*, void*, hipStream_t);
+template <class R> hipError_t launch_stage(const mfvm_grid*, const mfvm_ns_params*, const mfvm_stage*, hipStream_t);
+template <class R> hipError_t launch_cfl(const mfvm_grid*, const mfvm_ns_params*, const void*, double, double*, double*, hipStream_t);
+hipError_t launch_dt_finalize(const mfvm_grid*, const mfvm_ns_params*, const double*, double, double*, hipStream_t);
+long long ns_nblocks(const mfvm_grid*, const mfvm_ns_params*);
+template <class R> hipError_t launch_bcs(const mfvm_grid*, int, const int32_t*, const void*, void*, hipStream_t);
+template <class R> hipError_t launch_copy_interior(const mfvm_grid*, int, const void*, void*, hipStream_t);
+
+static thread_local std::string g_err;
+
+// ---- event timing ------------------------------------------------------------
+struct TimedLaunch {
+  int kind;
+  hipEvent_t a, b;
+};
+static thread_local bool g_timing = false;
+static thread_local std::vector<TimedLaunch> g_launches;
+static thread_local std::vector<hipEvent_t> g_event_pool;
+
+static hipEvent_t take_event() {
+  if (!g_event_pool.empty()) {
+    hipEvent_t e = g_event_pool.back();
+    g_event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+int t_begin(int kind, hipStream_t s) {
+  if (!g_timing) return -1;
+  TimedLaunch t;
+  t.kind = kind;
+  t.a = take_event();
+  t.b = take_event();
+  if (!t.a || !t.b) return -1;
+  hipEventRecord(t.a, s);
+  g_launches.push_back(t);
+  return (int)g_launches.size() - 1;
+}
+
+void t_end(int token, hipStream_t s) {
+  if (token < 0 || token >= (int)g_launches.size()) return;
+  hipEventRecord(g_launches[token].b, s);
+}
+
+static void t_recycle() {
+  for (auto& t : g_launches) {
+    g_event_pool.push_back(t.a);
+    g_event_pool.push_back(t.b);
+  }
+  g_launches.clear();
+}
+
+static int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+static int hip_check(hipError_t e, const char* where) {
+  if (e == hipSuccess) return MFVM_OK;
+  return fail(MFVM_E_HIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+Geo make_geo(const mfvm_grid* g) {
+  Geo o;
+  o.fac = g->fac;
+  for (int m = 0; m < MFVM_NMET; ++m)
+    for (int d = 0; d < 3; ++d) o.off[m][d] = g->off[m][d];
+  o.n1 = g->n[0];
+  o.n2 = g->n[1];
+  o.n3 = g->n[2];
+  o.curv = g->coord != MFVM_CART;
+  for (int d = 0; d < 3; ++d) {
+    o.lpow2[d] = g->lpow2[d];
+    o.linv[d] = g->lpow2[d] ? 1.0 / g->lval[d] : 0.0;
+  }
+  o.vpow2 = g->vpow2;
+  o.vinv = g->vpow2 ? 1.0 / g->vval : 0.0;
+  return o;
+}
+
+static int check_grid(const mfvm_grid* g) {
+  if (!g) return fail(MFVM_E_ARG, "grid descriptor is NULL");
+  for (int d = 0; d < 3; ++d)
+    if (g->n[d] < 1) return fail(MFVM_E_ARG, "grid sizes must be >= 1");
+  if (g->coord < MFVM_CART || g->coord > MFVM_SPH)
+    return fail(MFVM_E_ARG,
+                "Unknown coordinate system. Expected one of: cartesian, cylindrical, spherical.");
+  if (!g->fac) return fail(MFVM_E_ARG, "grid metric factors are NULL");
+  return MFVM_OK;
+}
+
+static int check_bc(int nvar, const int32_t* bc) {
+  if (nvar < 1 || nvar > MFVM_MAX_BCVARS) return fail(MFVM_E_ARG, "nvar out of range");
+  if (!bc) return fail(MFVM_E_ARG, "bc table is NULL");
+  for (int i = 0; i < nvar * 6; ++i)
+    if (bc[i] < MFVM_BC_NEUMANN || bc[i] > MFVM_BC_REFLECTING)
+      return fail(MFVM_E_ARG, "unknown boundary condition code");
+  return MFVM_OK;
+}
+
+static int check_ns(const mfvm_grid* g, const mfvm_ns_params* p) {
+  int rc = check_grid(g);
+  if (rc) return rc;
+  if (!p) return fail(MFVM_E_ARG, "params descriptor is NULL");
+  if (p->dtype != MFVM_F64 && p->dtype != MFVM_F32) return fail(MFVM_E_ARG, "dtype must be 0 (f64) or 1 (f32)");
+  if (p->limiter < MFVM_LIM_NONE || p->limiter > MFVM_LIM_VANLEER)
+    return fail(MFVM_E_UNSUP, "unsupported slope limiter / reconstructor");
+  if (p->riemann != MFVM_RS_HLL && p->riemann != MFVM_RS_RUSANOV)
+    return fail(MFVM_E_UNSUP, "unsupported Riemann solver");
+  const int bx = p->tile[0] > 0 ? p->tile[0] : 64, by = p->tile[1] > 0 ? p->tile[1] : 4;
+  if (bx < 2 || by < 1 || bx * by > 256 || (bx * by) % 64 != 0)
+    return fail(MFVM_E_ARG, "stage tile must have 2 <= x1 threads, x1*x2 threads a multiple of 64 and <= 256");
+  if (p->tile[2] < 0) return fail(MFVM_E_ARG, "stage tile x3 planes must be >= 0");
+  return check_bc(5, &p->bc[0][0]);
+}
+
+}  // namespace mfvm
+
+using namespace mfvm;
+
+extern "C" {
+
+const char* mfvm_last_error(void) { return g_err.c_str(); }
+
+int32_t mfvm_abi_version(void) { return 1; }
+
+int mfvm_timing(int32_t mode, double* ms, int64_t* count) {
+  if (mode == 0 || mode == 1) {
+    if (!g_launches.empty()) {
+      hipDeviceSynchronize();
+      t_recycle();
+    }
+    g_timing = mode == 1;
+    return MFVM_OK;
+  }
+  if (mode != 2 || !ms || !count) return fail(MFVM_E_ARG, "mfvm_timing: bad arguments");
+  for (int k = 0; k < MFVM_TK_N; ++k) {
+    ms[k] = 0.0;
+    count[k] = 0;
+  }
+  for (auto& t : g_launches) {
+    if (hipEventSynchronize(t.b) != hipSuccess) return fail(MFVM_E_HIP, "mfvm_timing: event sync failed");
+    float e = 0.f;
+    if (hipEventElapsedTime(&e, t.a, t.b) != hipSuccess) return fail(MFVM_E_HIP, "mfvm_timing: elapsed failed");
+    ms[t.kind] += e;
+    count[t.kind] += 1;
+  }
+  t_recycle();
+  return MFVM_OK;
+}
+
+int mfvm_apply_bcs(const mfvm_grid* g, int32_t nvar, const int32_t* bc, int32_t dtype, const void* src,
+                   void* dst, void* stream) {
+  int rc = check_grid(g);
+  if (!rc) rc = check_bc(nvar, bc);
+  if (rc) return rc;
+  if (!src || !dst) return fail(MFVM_E_ARG, "apply_bcs: NULL buffer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == MFVM_F64) return hip_check(launch_bcs<double>(g, nvar, bc, src, dst, s), "apply_bcs");
+  if (dtype == MFVM_F32) return hip_check(launch_bcs<float>(g, nvar, bc, src, dst, s), "apply_bcs");
+  return fail(MFVM_E_ARG, "dtype must be 0 (f64) or 1 (f32)");
+}
+
+int mfvm_copy_interior(const mfvm_grid* g, int32_t nvar, int32_t dtype, const void* src, void* dst,
+                       void* stream) {
+  int rc = check_grid(g);
+  if (rc) return rc;
+  if (nvar < 1 || !src || !dst) return fail(MFVM_E_ARG, "copy_interior: bad arguments");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == MFVM_F64) return hip_check(launch_copy_interior<double>(g, nvar, src, dst, s), "copy_interior");
+  if (dtype == MFVM_F32) return hip_check(launch_copy_interior<float>(g, nvar, src, dst, s), "copy_interior");
+  return fail(MFVM_E_ARG, "dtype must be 0 (f64) or 1 (f32)");
+}
+
+int mfvm_ns_closure(const mfvm_grid* g, const mfvm_ns_params* p, const void* U, void* C, void* stream) {
+  int rc = check_ns(g, p);
+  if (rc) return rc;
+  if (!U || !C) return fail(MFVM_E_ARG, "ns_closure: NULL buffer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return hip_check(p->dtype == MFVM_F64 ? launch_closure<double>(g, p, U, C, s)
+                                        : launch_closure<float>(g, p, U, C, s),
+                   "ns_closure");
+}
+
+int mfvm_ns_stage(const mfvm_grid* g, const mfvm_ns_params* p, const mfvm_stage* st, void* stream) {
+  int rc = check_ns(g, p);
+  if (rc) return rc;
+  if (!st || !st->U || !st->C) return fail(MFVM_E_ARG, "ns_stage: NULL input");
+  if (st->update < MFVM_UPD_NONE || st->update > MFVM_UPD_ACC6) return fail(MFVM_E_ARG, "ns_stage: bad update mode");
+  if (st->acc < MFVM_ACC_NONE || st->acc > MFVM_ACC_ADD) return fail(MFVM_E_ARG, "ns_stage: bad accumulator mode");
+  if (st->update != MFVM_UPD_NONE && !st->out) return fail(MFVM_E_ARG, "ns_stage: NULL output");
+  if ((st->update == MFVM_UPD_BASE || st->update == MFVM_UPD_SSP || st->update == MFVM_UPD_ACC6) && !st->U0)
+    return fail(MFVM_E_ARG, "ns_stage: NULL base state");
+  if ((st->acc != MFVM_ACC_NONE || st->update == MFVM_UPD_ACC6) && !st->kacc)
+    return fail(MFVM_E_ARG, "ns_stage: NULL accumulator");
+  if (st->update != MFVM_UPD_NONE && st->out == st->U)
+    return fail(MFVM_E_ARG, "ns_stage: output must not alias the stencil input");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return hip_check(p->dtype == MFVM_F64 ? launch_stage<double>(g, p, st, s) : launch_stage<float>(g, p, st, s),
+                   "ns_stage");
+}
+
+int mfvm_ns_rhs(const mfvm_grid* g, const mfvm_ns_params* p, const void* U, void* C, void* rhs, void* stream) {
+  int rc = mfvm_ns_closure(g, p, U, C, stream);
+  if (rc) return rc;
+  mfvm_stage st = {};
+  st.U = U;
+  st.C = C;
+  st.kacc = rhs;
+  st.update = MFVM_UPD_NONE;
+  st.acc = MFVM_ACC_SET;
+  return mfvm_ns_stage(g, p, &st, stream);
+}
+
+int64_t mfvm_ns_partials_size(const mfvm_grid* g, const mfvm_ns_params* p) {
+  if (check_ns(g, p)) return -1;
+  return 2 * ns_nblocks(g, p);
+}
+
+int mfvm_ns_dt(const mfvm_grid* g, const mfvm_ns_params* p, const void* U, double cfl, double* partials,
+               double* dt_out, void* stream) {
+  int rc = check_ns(g, p);
+  if (rc) return rc;
+  if (!U || !partials || !dt_out) return fail(MFVM_E_ARG, "ns_dt: NULL buffer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return hip_check(p->dtype == MFVM_F64 ? launch_cfl<double>(g, p, U, cfl, partials, dt_out, s)
+                                        : launch_cfl<float>(g, p, U, cfl, partials, dt_out, s),
+                   "ns_dt");
+}
+
+// Stage sequence of one integrator call (time_integrators.py).  dt from
+// dt_dev when non-NULL; CFL partials of the new state when partials != NULL.
+static int ns_step_impl(const mfvm_grid* g, const mfvm_ns_params* p, const mfvm_ns_work* w,
+                        int32_t integrator, const double* dt_dev, double dt, double* partials,
+                        bool fix_ghosts, void* stream) {
+  int rc;
+  const int32_t* bc = &p->bc[0][0];
+  struct S {
+    void* in;
+    void* out;
+    int update, acc;
+    double c0, c1, ac;
+  };
+  S seq[4];
+  int ns = 0;
+  void* last_in = nullptr;
+  switch (integrator) {
+    case MFVM_INT_EULER:  // time_integrators.py:43-47
+      seq[0] = {w->U, w->S1, MFVM_UPD_EULER, MFVM_ACC_NONE, 0, 0, 0};
+      ns = 1;
+      break;
+    case MFVM_INT_RK3:  // :87-103
+      seq[0] = {w->U, w->S1, MFVM_UPD_BASE, MFVM_ACC_SET, 0.5, 0, 0};
+      seq[1] = {w->S1, w->S2, MFVM_UPD_BASE, MFVM_ACC_ADD, 1.0, 0, 4.0};
+      seq[2] = {w->S2, w->U, MFVM_UPD_ACC6, MFVM_ACC_NONE, 0, 0, 0};
+      ns = 3;
+      last_in = w->S2;
+      break;
+    case MFVM_INT_SSPRK3:  // :143-159
+      seq[0] = {w->U, w->S1, MFVM_UPD_EULER, MFVM_ACC_NONE, 0, 0, 0};
+      seq[1] = {w->S1, w->S2, MFVM_UPD_SSP, MFVM_ACC_NONE, 0.75, 0.25, 0};
+      seq[2] = {w->S2, w->U, MFVM_UPD_SSP, MFVM_ACC_NONE, 1.0 / 3.0, 2.0 / 3.0, 0};
+      ns = 3;
+      last_in = w->S2;
+      break;
+    default:  // rk4 :199-221
+      seq[0] = {w->U, w->S1, MFVM_UPD_BASE, MFVM_ACC_SET, 0.5, 0, 0};
+      seq[1] = {w->S1, w->S2, MFVM_UPD_BASE, MFVM_ACC_ADD, 0.5, 0, 2.0};
+      seq[2] = {w->S2, w->S1, MFVM_UPD_BASE, MFVM_ACC_ADD, 1.0, 0, 2.0};
+      seq[3] = {w->S1, w->U, MFVM_UPD_ACC6, MFVM_ACC_NONE, 0, 0, 0};
+      ns = 4;
+      last_in = w->S1;
+      break;
+  }
+  for (int q = 0; q < ns; ++q) {
+    if ((rc = mfvm_apply_bcs(g, 5, bc, p->dtype, seq[q].in, seq[q].in, stream))) return rc;
+    if ((rc = mfvm_ns_closure(g, p, seq[q].in, w->C, stream))) return rc;
+    mfvm_stage st = {};
+    st.U = seq[q].in;
+    st.C = w->C;
+    st.U0 = w->U;
+    st.out = seq[q].out;
+    st.kacc = w->kacc;
+    st.update = seq[q].update;
+    st.acc = seq[q].acc;
+    st.c0 = seq[q].c0;
+    st.c1 = seq[q].c1;
+    st.acc_coef = seq[q].ac;
+    st.dt = dt;
+    st.dt_dev = dt_dev;
+    st.partials = (q == ns - 1) ? partials : nullptr;
+    if ((rc = mfvm_ns_stage(g, p, &st, stream))) return rc;
+  }
+  if (integrator == MFVM_INT_EULER && (rc = mfvm_copy_interior(g, 5, p->dtype, w->S1, w->U, stream)))
+    return rc;
+  // Leave the ghosts as the reference does: apply_bcs of the last stage input.
+  if (fix_ghosts && last_in && (rc = mfvm_apply_bcs(g, 5, bc, p->dtype, last_in, w->U, stream))) return rc;
+  return MFVM_OK;
+}
+
+static int check_work(const mfvm_ns_work* w, int32_t integrator) {
+  if (!w || !w->U || !w->S1 || !w->S2 || !w->C || !w->kacc)
+    return fail(MFVM_E_ARG, "ns work: NULL buffer");
+  if (integrator < MFVM_INT_EULER || integrator > MFVM_INT_RK4) return fail(MFVM_E_ARG, "unknown integrator");
+  return MFVM_OK;
+}
+
+int mfvm_ns_step(const mfvm_grid* g, const mfvm_ns_params* p, const mfvm_ns_work* w, int32_t integrator,
+                 double dt, void* stream) {
+  int rc = check_ns(g, p);
+  if (!rc) rc = check_work(w, integrator);
+  if (rc) return rc;
+  return ns_step_impl(g, p, w, integrator, nullptr, dt, nullptr, true, stream);
+}
+
+int mfvm_ns_advance(const mfvm_grid* g, const mfvm_ns_params* p, const mfvm_ns_work* w, int32_t integrator,
+                    double cfl, int32_t nsteps, void* stream) {
+  int rc = check_ns(g, p);
+  if (!rc) rc = check_work(w, integrator);
+  if (rc) return rc;
+  if (!w->partials || !w->dts) return fail(MFVM_E_ARG, "ns_advance: NULL partials/dts");
+  if (nsteps < 0) return fail(MFVM_E_ARG, "nsteps must be >= 0");
+  if (nsteps == 0) return MFVM_OK;
+  if ((rc = mfvm_ns_dt(g, p, w->U, cfl, w->partials, w->dts, stream))) return rc;
+  for (int n = 0; n < nsteps; ++n) {
+    if ((rc = ns_step_impl(g, p, w, integrator, w->dts + n, 0.0, w->partials, n == nsteps - 1, stream)))
+      return rc;
+    // dt of the next step from the new state (navier_stokes.py:512-585)
+    if ((rc = hip_check(launch_dt_finalize(g, p, w->partials, cfl, w->dts + n + 1,
+                                           static_cast<hipStream_t>(stream)),
+                        "ns_advance")))
+      return rc;
+  }
+  return MFVM_OK;
+}
+
+}  // extern "C"

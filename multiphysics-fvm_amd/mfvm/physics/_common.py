@@ -1,0 +1,72 @@
+"""Scheme resolution shared by the physics modules."""
+
+import ctypes
+
+import torch
+
+from .. import _lib
+from ..grid.boundary_conditions import bc_codes
+from ..state import State
+
+
+def require_state(U, names):
+    if not isinstance(U, State):
+        raise TypeError("U must be an mfvm State (use allocate_u); got " + type(U).__name__)
+    if U.names != tuple(names):
+        raise ValueError(f"U holds {U.names}, expected {tuple(names)}")
+    if U.device.type != "cuda":
+        raise RuntimeError("mfvm kernels need U on a HIP device")
+    return U
+
+
+def require_grid(grid, U):
+    g = grid.get("mfvm") if isinstance(grid, dict) else None
+    if g is None:
+        raise TypeError("grid must come from mfvm.grid.geometry.build_grid")
+    if tuple(g.n) != tuple(U.n):
+        raise ValueError(f"grid is {g.n}, state is {U.n}")
+    return g
+
+
+def limiter_code(space_schemes):
+    recon = space_schemes["reconstructor"]
+    kind = getattr(recon, "_mfvm_recon", None)
+    if kind is None:
+        raise NotImplementedError(f"reconstructor {recon!r} has no fused kernel")
+    if kind == "pc":
+        return _lib.LIM_NONE
+    lim = space_schemes["slope_limiter"]
+    code = getattr(lim, "_mfvm_code", None)
+    if code is None:
+        raise NotImplementedError(f"slope limiter {lim!r} has no fused kernel")
+    return code
+
+
+def riemann_code(space_schemes):
+    rs = space_schemes["riemann_solver"]
+    code = getattr(rs, "_mfvm_code", None)
+    if code is None:
+        raise NotImplementedError(f"Riemann solver {rs!r} has no fused kernel")
+    return code
+
+
+def dtype_code(U):
+    if U.dtype == torch.float64:
+        return _lib.F64
+    if U.dtype == torch.float32:
+        return _lib.F32
+    raise TypeError("state dtype must be float64 or float32")
+
+
+def bc_table(names, bcs):
+    rows = bc_codes(names, bcs)
+    flat = (ctypes.c_int32 * (6 * len(names)))(*[c for r in rows for c in r])
+    return rows, flat
+
+
+def ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def stream():
+    return _lib.stream_ptr(torch)

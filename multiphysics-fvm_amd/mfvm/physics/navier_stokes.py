@@ -1,0 +1,121 @@
+"""Compressible Navier-Stokes (drop-in for src/physics/navier_stokes.py).
+
+``compute_rhs`` and ``compute_dt`` run entirely in libmfvm.so (closure
+kernel, fused stage kernel in right-hand-side mode, CFL reduction); the
+integrators in :mod:`mfvm.numerics.time_integrators` recognise this module's
+``compute_rhs`` and run whole fused stages instead.
+"""
+
+import ctypes
+
+import torch
+
+from .. import _lib
+from ..state import State
+from . import _common as c
+
+VARS = ("mass_density", "momentum_x1", "momentum_x2", "momentum_x3", "energy_density")
+NCLOSURE = 7
+
+
+def allocate_u(grid_config, dtype=torch.float64, device="cuda"):
+    """navier_stokes.py:13-41 -- five zero fields with one ghost layer."""
+    return State(VARS, grid_config, dtype=dtype, device=device)
+
+
+def make_params(U, params, space_schemes, bcs=None, tile=None):
+    """C descriptor from the reference's params / space_schemes / bcs dicts."""
+    p = _lib.NSParams()
+    p.dtype = c.dtype_code(U)
+    p.limiter = c.limiter_code(space_schemes)
+    p.riemann = c.riemann_code(space_schemes)
+    ds = space_schemes["diffusive_solver"]
+    if getattr(ds, "_mfvm_code", None) != 0:
+        raise NotImplementedError(f"diffusive solver {ds!r} has no fused kernel")
+    p.gamma = float(params["heat_capacity_ratio"])
+    p.mu = float(params["dynamic_viscosity"])
+    g = params["gravitational_acceleration"]
+    p.g[:] = [float(g[0]), float(g[1]), float(g[2])]
+    if bcs is not None:
+        rows, _ = c.bc_table(VARS, bcs)
+        for v in range(5):
+            p.bc[v][:] = rows[v]
+    if tile is not None:
+        p.tile[:] = list(tile)
+    return p
+
+
+def _closure_buf(U):
+    n1, n2, n3 = U.n
+    return U.scratch("C", (NCLOSURE, n3 + 2, n2 + 2, n1 + 2), zero=True)
+
+
+def compute_rhs(U, grid, params, space_schemes):
+    """navier_stokes.py:588-711 -- returns fresh (nx1, nx2, nx3) tensors per field."""
+    c.require_state(U, VARS)
+    g = c.require_grid(grid, U)
+    p = make_params(U, params, space_schemes)
+    n1, n2, n3 = U.n
+    rhs = torch.empty((5, n3, n2, n1), dtype=U.dtype, device=U.device)
+    _lib.check(_lib.lib.mfvm_ns_rhs(g.ptr(), ctypes.byref(p), c.ptr(U.buf), c.ptr(_closure_buf(U)),
+                                    c.ptr(rhs), c.stream()))
+    return U.interior_views(rhs)
+
+
+compute_rhs._mfvm_physics = "navier_stokes"
+
+
+def compute_dt(U, params, grid, cfl):
+    """navier_stokes.py:512-585 -- CFL step as a Python float (one device sync)."""
+    c.require_state(U, VARS)
+    g = c.require_grid(grid, U)
+    p = _lib.NSParams()
+    p.dtype = c.dtype_code(U)
+    p.gamma = float(params["heat_capacity_ratio"])
+    p.mu = float(params["dynamic_viscosity"])
+    n = _lib.lib.mfvm_ns_partials_size(g.ptr(), ctypes.byref(p))
+    partials = U.scratch64("partials", max(int(n), 1))
+    dt = U.scratch64("dt", 1)
+    _lib.check(_lib.lib.mfvm_ns_dt(g.ptr(), ctypes.byref(p), c.ptr(U.buf), float(cfl),
+                                   c.ptr(partials), c.ptr(dt), c.stream()))
+    return float(dt[0].item())
+
+
+def work(U, nsteps=0):
+    """Scratch buffers of whole fused steps (mfvm_ns_work)."""
+    n1, n2, n3 = U.n
+    w = _lib.NSWork()
+    w.U = U.buf.data_ptr()
+    w.S1 = U.scratch("S1", tuple(U.buf.shape)).data_ptr()
+    w.S2 = U.scratch("S2", tuple(U.buf.shape)).data_ptr()
+    w.C = _closure_buf(U).data_ptr()
+    w.kacc = U.scratch("kacc", (5, n3, n2, n1)).data_ptr()
+    return w
+
+
+def fused_step(U, grid, params, space_schemes, bcs, integrator, dt):
+    """One integrator call through mfvm_ns_step."""
+    g = c.require_grid(grid, U)
+    p = make_params(U, params, space_schemes, bcs)
+    w = work(U)
+    _lib.check(_lib.lib.mfvm_ns_step(g.ptr(), ctypes.byref(p), ctypes.byref(w), integrator,
+                                     float(dt), c.stream()))
+
+
+def advance(U, grid, params, space_schemes, bcs, integrator, cfl, nsteps, tile=None):
+    """``nsteps`` iterations of the reference loop body (dt on the device).
+
+    Returns the device tensor of the nsteps+1 time steps (dts[s] used by step s).
+    """
+    c.require_state(U, VARS)
+    g = c.require_grid(grid, U)
+    p = make_params(U, params, space_schemes, bcs, tile)
+    w = work(U)
+    n = _lib.lib.mfvm_ns_partials_size(g.ptr(), ctypes.byref(p))
+    partials = U.scratch64("partials", max(int(n), 1))
+    dts = torch.zeros(nsteps + 1, dtype=torch.float64, device=U.device)
+    w.partials = partials.data_ptr()
+    w.dts = dts.data_ptr()
+    _lib.check(_lib.lib.mfvm_ns_advance(g.ptr(), ctypes.byref(p), ctypes.byref(w), integrator,
+                                        float(cfl), int(nsteps), c.stream()))
+    return dts
