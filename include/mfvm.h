@@ -95,6 +95,7 @@ typedef struct mfvm_grid {
   double lval[3];               /* that L_d (when lpow2[d]) */
   double vval;                  /* that V (when vpow2) */
   double smax;                  /* max over cells of 1/L1^2 + 1/L2^2 + 1/L3^2 */
+  double lmin[3];               /* min over cells of L_d */
 } mfvm_grid;
 
 typedef struct mfvm_ns_params {  /* navier_stokes.py: params + space_schemes */
@@ -133,11 +134,33 @@ typedef struct mfvm_stage {
   double* partials;             /* non-NULL: reduce CFL rates of out */
 } mfvm_stage;
 
-typedef struct mfvm_ns_work {   /* scratch for whole steps */
+typedef struct mfvm_heat_params {  /* heat.py: params + space_schemes */
+  int32_t dtype;
+  int32_t limiter;              /* MFVM_LIM_* (reconstruction of the heat flux) */
+  int32_t bc[1][6];
+  double rho_cp;                /* mass_density * specific_heat_capacity */
+  double conductivity;          /* thermal_conductivity */
+  double source;                /* volumetric_heat_source */
+} mfvm_heat_params;
+
+typedef struct mfvm_maxwell_params {  /* maxwell.py: params + space_schemes */
+  int32_t dtype;
+  int32_t limiter;
+  int32_t riemann;
+  int32_t bc[6][6];
+  double mu;                    /* magnetic_permeability */
+  double ep;                    /* electric_permittivity */
+  double sigma;                 /* electrical_conductivity */
+  double rho;                   /* charge_density */
+  double chi_b, chi_d;          /* magnetic / electric cleaning coefficients */
+} mfvm_maxwell_params;
+
+typedef struct mfvm_ns_work {   /* scratch for whole steps (all physics) */
   void* U;                      /* the state (ghosted), advanced in place */
   void* S1;                     /* ghosted stage states */
   void* S2;
-  void* C;                      /* ghosted closure, 7 fields; ghost entries must be 0 */
+  void* C;                      /* ghosted closure: NS 7 fields (ghost entries 0),
+                                   Maxwell 2 residual fields, heat unused */
   void* kacc;                   /* interior, 5 fields */
   double* partials;             /* >= mfvm_ns_partials_size() doubles */
   double* dts;                  /* device: dt of step s written to dts[s] */
@@ -212,6 +235,42 @@ int mfvm_ns_advance(const mfvm_grid* g, const mfvm_ns_params* p, const mfvm_ns_w
                     int32_t integrator, double cfl, int32_t nsteps, void* stream);
 
 int64_t mfvm_ns_partials_size(const mfvm_grid* g, const mfvm_ns_params* p);
+
+/* ---- heat equation (src/physics/heat.py) -------------------------------------- */
+
+/* Right-hand side into an interior buffer.  Replaces heat.py:354-473. */
+int mfvm_heat_rhs(const mfvm_grid* g, const mfvm_heat_params* p, const void* U, void* rhs,
+                  void* stream);
+/* dt (device scalar).  Replaces heat.py:305-351 (compute_dt). */
+int mfvm_heat_dt(const mfvm_grid* g, const mfvm_heat_params* p, double cfl, double* dt_out,
+                 void* stream);
+/* One integrator call (time_integrators.py with heat.compute_rhs). */
+int mfvm_heat_step(const mfvm_grid* g, const mfvm_heat_params* p, const mfvm_ns_work* w,
+                   int32_t integrator, double dt, void* stream);
+/* nsteps steps of solver.py:137-151; dts needs nsteps+1 entries. */
+int mfvm_heat_advance(const mfvm_grid* g, const mfvm_heat_params* p, const mfvm_ns_work* w,
+                      int32_t integrator, double cfl, int32_t nsteps, void* stream);
+
+/* ---- Maxwell (src/physics/maxwell.py) ------------------------------------------ */
+
+/* Cleaning residuals R = {div B, div D - rho} (2 ghosted fields, face ghosts
+ * edge-padded).  Replaces maxwell.py:491-512. */
+int mfvm_maxwell_residual(const mfvm_grid* g, const mfvm_maxwell_params* p, const void* U,
+                          void* R, void* stream);
+/* Right-hand side into an interior buffer; R is scratch.  Replaces
+ * maxwell.py:603-729 (compute_rhs). */
+int mfvm_maxwell_rhs(const mfvm_grid* g, const mfvm_maxwell_params* p, const void* U, void* R,
+                     void* rhs, void* stream);
+/* dt (device scalar).  Replaces maxwell.py:533-600 (compute_dt). */
+int mfvm_maxwell_dt(const mfvm_grid* g, const mfvm_maxwell_params* p, double cfl,
+                    double* dt_out, void* stream);
+/* One integrator call (time_integrators.py with maxwell.compute_rhs). */
+int mfvm_maxwell_step(const mfvm_grid* g, const mfvm_maxwell_params* p, const mfvm_ns_work* w,
+                      int32_t integrator, double dt, void* stream);
+/* nsteps steps of solver.py:137-151; dts needs nsteps+1 entries. */
+int mfvm_maxwell_advance(const mfvm_grid* g, const mfvm_maxwell_params* p,
+                         const mfvm_ns_work* w, int32_t integrator, double cfl,
+                         int32_t nsteps, void* stream);
 
 #ifdef __cplusplus
 }

@@ -6,7 +6,7 @@
 
 namespace mfvm {
 
-// Defined in ns.hip / bc.hip.
+// Defined in ns.hip / bc.hip / heat.hip / maxwell.hip.
 template <class R> hipError_t launch_closure(const mfvm_grid*, const mfvm_ns_params*, const void*, void*, hipStream_t);
 template <class R> hipError_t launch_stage(const mfvm_grid*, const mfvm_ns_params*, const mfvm_stage*, hipStream_t);
 template <class R> hipError_t launch_cfl(const mfvm_grid*, const mfvm_ns_params*, const void*, double, double*, double*, hipStream_t);
@@ -14,6 +14,11 @@ hipError_t launch_dt_finalize(const mfvm_grid*, const mfvm_ns_params*, const dou
 long long ns_nblocks(const mfvm_grid*, const mfvm_ns_params*);
 template <class R> hipError_t launch_bcs(const mfvm_grid*, int, const int32_t*, const void*, void*, hipStream_t);
 template <class R> hipError_t launch_copy_interior(const mfvm_grid*, int, const void*, void*, hipStream_t);
+template <class R> hipError_t heat_launch_stage(const mfvm_grid*, const mfvm_heat_params*, const void*, const StageIO&, hipStream_t);
+hipError_t heat_launch_dt(const mfvm_grid*, const mfvm_heat_params*, double, double*, hipStream_t);
+template <class R> hipError_t maxwell_launch_res(const mfvm_grid*, const mfvm_maxwell_params*, const void*, void*, hipStream_t);
+template <class R> hipError_t maxwell_launch_stage(const mfvm_grid*, const mfvm_maxwell_params*, const void*, const void*, const StageIO&, hipStream_t);
+hipError_t maxwell_launch_dt(const mfvm_grid*, const mfvm_maxwell_params*, double, double*, hipStream_t);
 
 static thread_local std::string g_err;
 
@@ -240,13 +245,22 @@ int mfvm_ns_dt(const mfvm_grid* g, const mfvm_ns_params* p, const void* U, doubl
                    "ns_dt");
 }
 
-// Stage sequence of one integrator call (time_integrators.py).  dt from
-// dt_dev when non-NULL; CFL partials of the new state when partials != NULL.
-static int ns_step_impl(const mfvm_grid* g, const mfvm_ns_params* p, const mfvm_ns_work* w,
-                        int32_t integrator, const double* dt_dev, double dt, double* partials,
-                        bool fix_ghosts, void* stream) {
+// Stage sequence of one integrator call (time_integrators.py) for any
+// physics: per stage a ghost fill of the stage input, the physics'
+// pre-pass (closure / residual), and one fused stage launch.  dt from dt_dev
+// when non-NULL; CFL partials of the new state when partials != NULL.
+struct PhysOps {
+  int nvar;
+  int dtype;
+  const int32_t* bc;
+  void* ctx;
+  int (*pre)(void* ctx, const void* in, void* C, void* stream);
+  int (*stage)(void* ctx, const void* in, const void* C, const StageIO& io, double* partials, void* stream);
+};
+
+static int step_impl(const mfvm_grid* g, const PhysOps& ph, const mfvm_ns_work* w, int32_t integrator,
+                     const double* dt_dev, double dt, double* partials, bool fix_ghosts, void* stream) {
   int rc;
-  const int32_t* bc = &p->bc[0][0];
   struct S {
     void* in;
     void* out;
@@ -285,57 +299,87 @@ static int ns_step_impl(const mfvm_grid* g, const mfvm_ns_params* p, const mfvm_
       break;
   }
   for (int q = 0; q < ns; ++q) {
-    if ((rc = mfvm_apply_bcs(g, 5, bc, p->dtype, seq[q].in, seq[q].in, stream))) return rc;
-    if ((rc = mfvm_ns_closure(g, p, seq[q].in, w->C, stream))) return rc;
-    mfvm_stage st = {};
-    st.U = seq[q].in;
-    st.C = w->C;
-    st.U0 = w->U;
-    st.out = seq[q].out;
-    st.kacc = w->kacc;
-    st.update = seq[q].update;
-    st.acc = seq[q].acc;
-    st.c0 = seq[q].c0;
-    st.c1 = seq[q].c1;
-    st.acc_coef = seq[q].ac;
-    st.dt = dt;
-    st.dt_dev = dt_dev;
-    st.partials = (q == ns - 1) ? partials : nullptr;
-    if ((rc = mfvm_ns_stage(g, p, &st, stream))) return rc;
+    if ((rc = mfvm_apply_bcs(g, ph.nvar, ph.bc, ph.dtype, seq[q].in, seq[q].in, stream))) return rc;
+    if (ph.pre && (rc = ph.pre(ph.ctx, seq[q].in, w->C, stream))) return rc;
+    StageIO io;
+    io.U0 = w->U;
+    io.out = seq[q].out;
+    io.kacc = w->kacc;
+    io.update = seq[q].update;
+    io.acc = seq[q].acc;
+    io.c0 = seq[q].c0;
+    io.c1 = seq[q].c1;
+    io.acc_coef = seq[q].ac;
+    io.dt = dt;
+    io.dt_dev = dt_dev;
+    if ((rc = ph.stage(ph.ctx, seq[q].in, w->C, io, (q == ns - 1) ? partials : nullptr, stream))) return rc;
   }
-  if (integrator == MFVM_INT_EULER && (rc = mfvm_copy_interior(g, 5, p->dtype, w->S1, w->U, stream)))
+  if (integrator == MFVM_INT_EULER && (rc = mfvm_copy_interior(g, ph.nvar, ph.dtype, w->S1, w->U, stream)))
     return rc;
   // Leave the ghosts as the reference does: apply_bcs of the last stage input.
-  if (fix_ghosts && last_in && (rc = mfvm_apply_bcs(g, 5, bc, p->dtype, last_in, w->U, stream))) return rc;
+  if (fix_ghosts && last_in && (rc = mfvm_apply_bcs(g, ph.nvar, ph.bc, ph.dtype, last_in, w->U, stream)))
+    return rc;
   return MFVM_OK;
 }
 
-static int check_work(const mfvm_ns_work* w, int32_t integrator) {
-  if (!w || !w->U || !w->S1 || !w->S2 || !w->C || !w->kacc)
-    return fail(MFVM_E_ARG, "ns work: NULL buffer");
+static int check_work(const mfvm_ns_work* w, int32_t integrator, bool need_c) {
+  if (!w || !w->U || !w->S1 || !w->S2 || !w->kacc || (need_c && !w->C))
+    return fail(MFVM_E_ARG, "work: NULL buffer");
   if (integrator < MFVM_INT_EULER || integrator > MFVM_INT_RK4) return fail(MFVM_E_ARG, "unknown integrator");
   return MFVM_OK;
+}
+
+// ---- Navier-Stokes ops ----
+struct NSCtx {
+  const mfvm_grid* g;
+  const mfvm_ns_params* p;
+};
+static int ns_pre(void* c, const void* in, void* C, void* stream) {
+  NSCtx* x = static_cast<NSCtx*>(c);
+  return mfvm_ns_closure(x->g, x->p, in, C, stream);
+}
+static int ns_stage_op(void* c, const void* in, const void* C, const StageIO& io, double* partials, void* stream) {
+  NSCtx* x = static_cast<NSCtx*>(c);
+  mfvm_stage st = {};
+  st.U = in;
+  st.C = C;
+  st.U0 = io.U0;
+  st.out = io.out;
+  st.kacc = io.kacc;
+  st.update = io.update;
+  st.acc = io.acc;
+  st.c0 = io.c0;
+  st.c1 = io.c1;
+  st.acc_coef = io.acc_coef;
+  st.dt = io.dt;
+  st.dt_dev = io.dt_dev;
+  st.partials = partials;
+  return mfvm_ns_stage(x->g, x->p, &st, stream);
 }
 
 int mfvm_ns_step(const mfvm_grid* g, const mfvm_ns_params* p, const mfvm_ns_work* w, int32_t integrator,
                  double dt, void* stream) {
   int rc = check_ns(g, p);
-  if (!rc) rc = check_work(w, integrator);
+  if (!rc) rc = check_work(w, integrator, true);
   if (rc) return rc;
-  return ns_step_impl(g, p, w, integrator, nullptr, dt, nullptr, true, stream);
+  NSCtx x{g, p};
+  PhysOps ph{5, p->dtype, &p->bc[0][0], &x, ns_pre, ns_stage_op};
+  return step_impl(g, ph, w, integrator, nullptr, dt, nullptr, true, stream);
 }
 
 int mfvm_ns_advance(const mfvm_grid* g, const mfvm_ns_params* p, const mfvm_ns_work* w, int32_t integrator,
                     double cfl, int32_t nsteps, void* stream) {
   int rc = check_ns(g, p);
-  if (!rc) rc = check_work(w, integrator);
+  if (!rc) rc = check_work(w, integrator, true);
   if (rc) return rc;
   if (!w->partials || !w->dts) return fail(MFVM_E_ARG, "ns_advance: NULL partials/dts");
   if (nsteps < 0) return fail(MFVM_E_ARG, "nsteps must be >= 0");
   if (nsteps == 0) return MFVM_OK;
   if ((rc = mfvm_ns_dt(g, p, w->U, cfl, w->partials, w->dts, stream))) return rc;
+  NSCtx x{g, p};
+  PhysOps ph{5, p->dtype, &p->bc[0][0], &x, ns_pre, ns_stage_op};
   for (int n = 0; n < nsteps; ++n) {
-    if ((rc = ns_step_impl(g, p, w, integrator, w->dts + n, 0.0, w->partials, n == nsteps - 1, stream)))
+    if ((rc = step_impl(g, ph, w, integrator, w->dts + n, 0.0, w->partials, n == nsteps - 1, stream)))
       return rc;
     // dt of the next step from the new state (navier_stokes.py:512-585)
     if ((rc = hip_check(launch_dt_finalize(g, p, w->partials, cfl, w->dts + n + 1,
@@ -343,6 +387,161 @@ int mfvm_ns_advance(const mfvm_grid* g, const mfvm_ns_params* p, const mfvm_ns_w
                         "ns_advance")))
       return rc;
   }
+  return MFVM_OK;
+}
+
+// ---- heat ----
+static int check_heat(const mfvm_grid* g, const mfvm_heat_params* p) {
+  int rc = check_grid(g);
+  if (rc) return rc;
+  if (!p) return fail(MFVM_E_ARG, "params descriptor is NULL");
+  if (p->dtype != MFVM_F64 && p->dtype != MFVM_F32) return fail(MFVM_E_ARG, "dtype must be 0 (f64) or 1 (f32)");
+  if (p->limiter < MFVM_LIM_NONE || p->limiter > MFVM_LIM_VANLEER)
+    return fail(MFVM_E_UNSUP, "unsupported slope limiter / reconstructor");
+  return check_bc(1, &p->bc[0][0]);
+}
+struct HeatCtx {
+  const mfvm_grid* g;
+  const mfvm_heat_params* p;
+};
+static int heat_stage_op(void* c, const void* in, const void*, const StageIO& io, double*, void* stream) {
+  HeatCtx* x = static_cast<HeatCtx*>(c);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return hip_check(x->p->dtype == MFVM_F64 ? heat_launch_stage<double>(x->g, x->p, in, io, s)
+                                           : heat_launch_stage<float>(x->g, x->p, in, io, s),
+                   "heat_stage");
+}
+
+int mfvm_heat_rhs(const mfvm_grid* g, const mfvm_heat_params* p, const void* U, void* rhs, void* stream) {
+  int rc = check_heat(g, p);
+  if (rc) return rc;
+  if (!U || !rhs) return fail(MFVM_E_ARG, "heat_rhs: NULL buffer");
+  HeatCtx x{g, p};
+  StageIO io = {};
+  io.kacc = rhs;
+  io.update = MFVM_UPD_NONE;
+  io.acc = MFVM_ACC_SET;
+  return heat_stage_op(&x, U, nullptr, io, nullptr, stream);
+}
+
+int mfvm_heat_dt(const mfvm_grid* g, const mfvm_heat_params* p, double cfl, double* dt_out, void* stream) {
+  int rc = check_heat(g, p);
+  if (rc) return rc;
+  if (!dt_out) return fail(MFVM_E_ARG, "heat_dt: NULL output");
+  return hip_check(heat_launch_dt(g, p, cfl, dt_out, static_cast<hipStream_t>(stream)), "heat_dt");
+}
+
+int mfvm_heat_step(const mfvm_grid* g, const mfvm_heat_params* p, const mfvm_ns_work* w, int32_t integrator,
+                   double dt, void* stream) {
+  int rc = check_heat(g, p);
+  if (!rc) rc = check_work(w, integrator, false);
+  if (rc) return rc;
+  HeatCtx x{g, p};
+  PhysOps ph{1, p->dtype, &p->bc[0][0], &x, nullptr, heat_stage_op};
+  return step_impl(g, ph, w, integrator, nullptr, dt, nullptr, true, stream);
+}
+
+int mfvm_heat_advance(const mfvm_grid* g, const mfvm_heat_params* p, const mfvm_ns_work* w, int32_t integrator,
+                      double cfl, int32_t nsteps, void* stream) {
+  int rc = check_heat(g, p);
+  if (!rc) rc = check_work(w, integrator, false);
+  if (rc) return rc;
+  if (!w->dts || nsteps < 0) return fail(MFVM_E_ARG, "heat_advance: bad dts / nsteps");
+  HeatCtx x{g, p};
+  PhysOps ph{1, p->dtype, &p->bc[0][0], &x, nullptr, heat_stage_op};
+  for (int n = 0; n < nsteps; ++n) {
+    // heat dt depends on the grid only (heat.py:338-349)
+    if ((rc = mfvm_heat_dt(g, p, cfl, w->dts + n, stream))) return rc;
+    if ((rc = step_impl(g, ph, w, integrator, w->dts + n, 0.0, nullptr, n == nsteps - 1, stream))) return rc;
+  }
+  if (nsteps > 0 && (rc = mfvm_heat_dt(g, p, cfl, w->dts + nsteps, stream))) return rc;
+  return MFVM_OK;
+}
+
+// ---- Maxwell ----
+static int check_mx(const mfvm_grid* g, const mfvm_maxwell_params* p) {
+  int rc = check_grid(g);
+  if (rc) return rc;
+  if (!p) return fail(MFVM_E_ARG, "params descriptor is NULL");
+  if (p->dtype != MFVM_F64 && p->dtype != MFVM_F32) return fail(MFVM_E_ARG, "dtype must be 0 (f64) or 1 (f32)");
+  if (p->limiter < MFVM_LIM_NONE || p->limiter > MFVM_LIM_VANLEER)
+    return fail(MFVM_E_UNSUP, "unsupported slope limiter / reconstructor");
+  if (p->riemann != MFVM_RS_HLL && p->riemann != MFVM_RS_RUSANOV)
+    return fail(MFVM_E_UNSUP, "unsupported Riemann solver");
+  if (!(p->mu > 0.0) || !(p->ep > 0.0)) return fail(MFVM_E_ARG, "permeability and permittivity must be > 0");
+  return check_bc(6, &p->bc[0][0]);
+}
+struct MXCtx {
+  const mfvm_grid* g;
+  const mfvm_maxwell_params* p;
+};
+static int mx_pre(void* c, const void* in, void* R, void* stream) {
+  MXCtx* x = static_cast<MXCtx*>(c);
+  return mfvm_maxwell_residual(x->g, x->p, in, R, stream);
+}
+static int mx_stage_op(void* c, const void* in, const void* R, const StageIO& io, double*, void* stream) {
+  MXCtx* x = static_cast<MXCtx*>(c);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return hip_check(x->p->dtype == MFVM_F64 ? maxwell_launch_stage<double>(x->g, x->p, in, R, io, s)
+                                           : maxwell_launch_stage<float>(x->g, x->p, in, R, io, s),
+                   "maxwell_stage");
+}
+
+int mfvm_maxwell_residual(const mfvm_grid* g, const mfvm_maxwell_params* p, const void* U, void* R,
+                          void* stream) {
+  int rc = check_mx(g, p);
+  if (rc) return rc;
+  if (!U || !R) return fail(MFVM_E_ARG, "maxwell_residual: NULL buffer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return hip_check(p->dtype == MFVM_F64 ? maxwell_launch_res<double>(g, p, U, R, s)
+                                        : maxwell_launch_res<float>(g, p, U, R, s),
+                   "maxwell_residual");
+}
+
+int mfvm_maxwell_rhs(const mfvm_grid* g, const mfvm_maxwell_params* p, const void* U, void* R, void* rhs,
+                     void* stream) {
+  int rc = mfvm_maxwell_residual(g, p, U, R, stream);
+  if (rc) return rc;
+  if (!rhs) return fail(MFVM_E_ARG, "maxwell_rhs: NULL output");
+  MXCtx x{g, p};
+  StageIO io = {};
+  io.kacc = rhs;
+  io.update = MFVM_UPD_NONE;
+  io.acc = MFVM_ACC_SET;
+  return mx_stage_op(&x, U, R, io, nullptr, stream);
+}
+
+int mfvm_maxwell_dt(const mfvm_grid* g, const mfvm_maxwell_params* p, double cfl, double* dt_out, void* stream) {
+  int rc = check_mx(g, p);
+  if (rc) return rc;
+  if (!dt_out) return fail(MFVM_E_ARG, "maxwell_dt: NULL output");
+  return hip_check(maxwell_launch_dt(g, p, cfl, dt_out, static_cast<hipStream_t>(stream)), "maxwell_dt");
+}
+
+int mfvm_maxwell_step(const mfvm_grid* g, const mfvm_maxwell_params* p, const mfvm_ns_work* w,
+                      int32_t integrator, double dt, void* stream) {
+  int rc = check_mx(g, p);
+  if (!rc) rc = check_work(w, integrator, true);
+  if (rc) return rc;
+  MXCtx x{g, p};
+  PhysOps ph{6, p->dtype, &p->bc[0][0], &x, mx_pre, mx_stage_op};
+  return step_impl(g, ph, w, integrator, nullptr, dt, nullptr, true, stream);
+}
+
+int mfvm_maxwell_advance(const mfvm_grid* g, const mfvm_maxwell_params* p, const mfvm_ns_work* w,
+                         int32_t integrator, double cfl, int32_t nsteps, void* stream) {
+  int rc = check_mx(g, p);
+  if (!rc) rc = check_work(w, integrator, true);
+  if (rc) return rc;
+  if (!w->dts || nsteps < 0) return fail(MFVM_E_ARG, "maxwell_advance: bad dts / nsteps");
+  MXCtx x{g, p};
+  PhysOps ph{6, p->dtype, &p->bc[0][0], &x, mx_pre, mx_stage_op};
+  for (int n = 0; n < nsteps; ++n) {
+    // Maxwell dt depends on the grid and parameters only (maxwell.py:579-598)
+    if ((rc = mfvm_maxwell_dt(g, p, cfl, w->dts + n, stream))) return rc;
+    if ((rc = step_impl(g, ph, w, integrator, w->dts + n, 0.0, nullptr, n == nsteps - 1, stream))) return rc;
+  }
+  if (nsteps > 0 && (rc = mfvm_maxwell_dt(g, p, cfl, w->dts + nsteps, stream))) return rc;
   return MFVM_OK;
 }
 

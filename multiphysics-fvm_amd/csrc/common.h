@@ -110,6 +110,50 @@ inline ILayout make_ilayout(int n1, int n2, int n3) {
 
 Geo make_geo(const mfvm_grid* g);
 
+// Runge-Kutta stage update of one value (time_integrators.py), k = rhs.
+//   EULER: U + dt*k                 BASE: U0 + (c0*dt)*k
+//   SSP:   c0*U0 + c1*(U + dt*k)     ACC6: U0 + (dt/6)*(kacc + k)
+template <class R>
+MFVM_DEV R stage_value(int mode, R u, R u0, R k, R ka, double c0, double c1, double dt) {
+  switch (mode) {
+    case MFVM_UPD_EULER: return u + R(dt) * k;
+    case MFVM_UPD_BASE: return u0 + R(c0 * dt) * k;
+    case MFVM_UPD_SSP: return (R(c0) * u0) + (R(c1) * (u + R(dt) * k));
+    case MFVM_UPD_ACC6: return u0 + R(dt / 6.0) * (ka + k);
+    default: return u;
+  }
+}
+
+// Common stage arguments of every physics (update and accumulator modes).
+struct StageIO {
+  const void* U0;
+  void* out;
+  void* kacc;
+  int update, acc;
+  double c0, c1, acc_coef, dt;
+  const double* dt_dev;
+};
+
+// Apply the stage update and accumulator of `nv` fields at one cell.
+template <class R>
+MFVM_DEV void stage_store(const StageIO& io, int nv, long long fs, long long ifs, long long oc,
+                          long long oi, const R* u, const R* k, R* newu) {
+  const double dt = io.dt_dev ? *io.dt_dev : io.dt;
+  R* out = static_cast<R*>(io.out);
+  R* ka = static_cast<R*>(io.kacc);
+  const R* U0 = static_cast<const R*>(io.U0);
+  for (int q = 0; q < nv; ++q) {
+    const bool need0 = io.update == MFVM_UPD_BASE || io.update == MFVM_UPD_SSP || io.update == MFVM_UPD_ACC6;
+    const R u0 = need0 ? U0[oc + q * fs] : R(0);
+    const R kav = io.update == MFVM_UPD_ACC6 ? ka[oi + q * ifs] : R(0);
+    const R o = stage_value<R>(io.update, u[q], u0, k[q], kav, io.c0, io.c1, dt);
+    if (newu) newu[q] = o;
+    if (io.update != MFVM_UPD_NONE) out[oc + q * fs] = o;
+    if (io.acc == MFVM_ACC_SET) ka[oi + q * ifs] = k[q];
+    else if (io.acc == MFVM_ACC_ADD) ka[oi + q * ifs] = ka[oi + q * ifs] + R(io.acc_coef) * k[q];
+  }
+}
+
 // Event timing around launches (mfvm_timing).  t_begin returns a token that
 // t_end closes on the same stream; both are no-ops while timing is off.
 int t_begin(int kind, hipStream_t s);

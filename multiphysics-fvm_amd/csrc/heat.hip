@@ -1,0 +1,174 @@
+// Heat equation kernels (reference src/physics/heat.py).
+//
+// One thread per interior cell.  The Fourier flux phi_d = -k dT/dx_d is a
+// centred difference of T = e/(rho cp) on interior cells and exactly zero on
+// ghost cells (the reference zero-pads the closure, heat.py:95); only phi_d
+// is reconstructed along d (the face normal picks that component).  The face
+// flux is the centred mean of the two reconstructed values
+// (diffusive_solvers.py:28), F = 0.0 + Fd (heat.py:446-448), and
+// rhs = -sum(F.S)/V + q (heat.py:471).  Neighbouring phi values are
+// recomputed per thread (a 2-deep stencil of T) rather than staged: the
+// update is bandwidth-bound and the recomputation is cheap.
+#include "common.h"
+
+namespace mfvm {
+
+struct HeatArgs {
+  Layout lay;
+  ILayout il;
+  Geo geo;
+  const void* U;
+  StageIO io;
+  double inv_rcp;   // unused when rcp_pow2 == 0
+  int rcp_pow2;     // rho*cp is a power of two: T = e * (1/(rho cp)) exactly
+  double rcp;       // rho*cp
+  double negk;      // -k
+  double q;         // volumetric source
+};
+
+template <class R>
+MFVM_DEV R heat_T(const HeatArgs& a, const R* U, long long o) {
+  return a.rcp_pow2 ? U[o] * R(a.inv_rcp) : U[o] / R(a.rcp);
+}
+
+// phi_d at ghosted cell (i,j,k): zero outside the interior along d.
+template <int D, class R>
+MFVM_DEV R heat_phi(const HeatArgs& a, const R* U, int i, int j, int k) {
+  const int c = D == 0 ? i : (D == 1 ? j : k);
+  const int n = D == 0 ? a.lay.n1 : (D == 1 ? a.lay.n2 : a.lay.n3);
+  if (c < 1 || c > n) return R(0);
+  const long long st = D == 0 ? 1 : (D == 1 ? a.lay.sy : a.lay.sz);
+  const long long o = a.lay.at(i, j, k);
+  const R dT = heat_T<R>(a, U, o + st) - heat_T<R>(a, U, o - st);
+  R g;
+  if (a.geo.lpow2[D]) g = dT * R(0.5 * a.geo.linv[D]);
+  else g = dT / (R(2) * met<R>(a.geo, MFVM_MET_L1 + D, i - 1, j - 1, k - 1));
+  return R(a.negk) * g;  // heat.py:85-92
+}
+
+template <int D, int LIM, class R>
+MFVM_DEV R heat_slope(const HeatArgs& a, const R* U, int i, int j, int k, R pm, R p, R pp) {
+  const int c = D == 0 ? i : (D == 1 ? j : k);
+  const int n = D == 0 ? a.lay.n1 : (D == 1 ? a.lay.n2 : a.lay.n3);
+  if (LIM == MFVM_LIM_NONE || c < 1 || c > n) return R(0);
+  const R len = a.geo.lpow2[D] ? R(1.0 / a.geo.linv[D]) : met<R>(a.geo, MFVM_MET_L1 + D, i - 1, j - 1, k - 1);
+  const R b = div_len<R>(a.geo, D, p - pm, len);
+  const R f = div_len<R>(a.geo, D, pp - p, len);
+  return limit<LIM, R>(b, f) * len;
+}
+
+// Face fluxes F.S on the lower and upper D-faces of interior cell (i,j,k).
+template <int D, int LIM, class R>
+MFVM_DEV void heat_dir(const HeatArgs& a, const R* U, int i, int j, int k, R& fs_lo, R& fs_hi) {
+  auto at = [&](int t, int& x, int& y, int& z) {
+    x = i + (D == 0 ? t : 0);
+    y = j + (D == 1 ? t : 0);
+    z = k + (D == 2 ? t : 0);
+  };
+  const int c = D == 0 ? i : (D == 1 ? j : k);
+  const int n = D == 0 ? a.lay.n1 : (D == 1 ? a.lay.n2 : a.lay.n3);
+  R ph[5];
+  int x, y, z;
+#pragma unroll
+  for (int t = -2; t <= 2; ++t) {
+    const int cc = c + t;
+    if (cc < 0 || cc > n + 1 || (LIM == MFVM_LIM_NONE && (t == -2 || t == 2))) {
+      ph[t + 2] = R(0);
+      continue;
+    }
+    at(t, x, y, z);
+    ph[t + 2] = heat_phi<D, R>(a, U, x, y, z);
+  }
+  R s[3];
+#pragma unroll
+  for (int t = -1; t <= 1; ++t) {
+    at(t, x, y, z);
+    s[t + 1] = heat_slope<D, LIM, R>(a, U, x, y, z, ph[t + 1], ph[t + 2], ph[t + 3]);
+  }
+  // lower face: L from cell c-1 (upper side), R from cell c (lower side)
+  const R lo_l = ph[1] + s[0] / R(2), lo_r = ph[2] - s[1] / R(2);
+  const R hi_l = ph[2] + s[1] / R(2), hi_r = ph[3] - s[2] / R(2);
+  const R f_lo = R(0) + (lo_l + lo_r) / R(2);
+  const R f_hi = R(0) + (hi_l + hi_r) / R(2);
+  const int m = MFVM_MET_S1 + D;
+  if (D == 0) {
+    fs_lo = f_lo * met<R>(a.geo, m, i - 1, j - 1, k - 1);
+    fs_hi = f_hi * met<R>(a.geo, m, i, j - 1, k - 1);
+  } else if (D == 1) {
+    fs_lo = f_lo * met<R>(a.geo, m, i - 1, j - 1, k - 1);
+    fs_hi = f_hi * met<R>(a.geo, m, i - 1, j, k - 1);
+  } else {
+    fs_lo = f_lo * met<R>(a.geo, m, i - 1, j - 1, k - 1);
+    fs_hi = f_hi * met<R>(a.geo, m, i - 1, j - 1, k);
+  }
+}
+
+template <int LIM, class R>
+__global__ __launch_bounds__(256) void heat_stage_kernel(HeatArgs a) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > a.lay.n1 || j > a.lay.n2) return;
+  const R* U = static_cast<const R*>(a.U);
+  R l1, h1, l2, h2, l3, h3;
+  heat_dir<0, LIM, R>(a, U, i, j, k, l1, h1);
+  heat_dir<1, LIM, R>(a, U, i, j, k, l2, h2);
+  heat_dir<2, LIM, R>(a, U, i, j, k, l3, h3);
+  const R x = -(((((h1 - l1) + h2) - l2) + h3) - l3);
+  R kr = a.geo.vpow2 ? x * R(a.geo.vinv) : x / met<R>(a.geo, MFVM_MET_V, i - 1, j - 1, k - 1);
+  kr = kr + R(a.q);
+  const long long oc = a.lay.at(i, j, k);
+  const R u = U[oc];
+  stage_store<R>(a.io, 1, a.lay.fs, a.il.fs, oc, a.il.at(i, j, k), &u, &kr, nullptr);
+}
+
+// dt = cfl / (0.0 + (2*nu)*smax), nu = k/(rho cp)   (heat.py:338-349)
+__global__ void heat_dt_kernel(double nu, double smax, double cfl, double* dt_out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *dt_out = cfl / (0.0 + (2.0 * nu) * smax);
+}
+
+template <int LIM, class R>
+static hipError_t heat_launch_t(const HeatArgs& a, hipStream_t s) {
+  const dim3 blk(64, 4, 1);
+  const dim3 grd((a.lay.n1 + 63) / 64, (a.lay.n2 + 3) / 4, a.lay.n3);
+  const int tk = t_begin(MFVM_TK_STAGE, s);
+  hipLaunchKernelGGL((heat_stage_kernel<LIM, R>), grd, blk, 0, s, a);
+  t_end(tk, s);
+  return hipGetLastError();
+}
+
+template <class R>
+hipError_t heat_launch_stage(const mfvm_grid* g, const mfvm_heat_params* p, const void* U,
+                             const StageIO& io, hipStream_t s) {
+  HeatArgs a;
+  a.lay = make_layout(g->n[0], g->n[1], g->n[2]);
+  a.il = make_ilayout(g->n[0], g->n[1], g->n[2]);
+  a.geo = make_geo(g);
+  a.U = U;
+  a.io = io;
+  a.rcp = p->rho_cp;
+  int e = 0;
+  const double m = frexp(p->rho_cp, &e);
+  a.rcp_pow2 = (m == 0.5);
+  a.inv_rcp = a.rcp_pow2 ? 1.0 / p->rho_cp : 0.0;
+  a.negk = -p->conductivity;
+  a.q = p->source;
+  switch (p->limiter) {
+    case MFVM_LIM_NONE: return heat_launch_t<MFVM_LIM_NONE, R>(a, s);
+    case MFVM_LIM_MINMOD: return heat_launch_t<MFVM_LIM_MINMOD, R>(a, s);
+    case MFVM_LIM_MC: return heat_launch_t<MFVM_LIM_MC, R>(a, s);
+    default: return heat_launch_t<MFVM_LIM_VANLEER, R>(a, s);
+  }
+}
+
+hipError_t heat_launch_dt(const mfvm_grid* g, const mfvm_heat_params* p, double cfl, double* dt_out,
+                          hipStream_t s) {
+  const double nu = p->conductivity / p->rho_cp;
+  hipLaunchKernelGGL(heat_dt_kernel, dim3(1), dim3(64), 0, s, nu, g->smax, cfl, dt_out);
+  return hipGetLastError();
+}
+
+template hipError_t heat_launch_stage<double>(const mfvm_grid*, const mfvm_heat_params*, const void*, const StageIO&, hipStream_t);
+template hipError_t heat_launch_stage<float>(const mfvm_grid*, const mfvm_heat_params*, const void*, const StageIO&, hipStream_t);
+
+}  // namespace mfvm

@@ -1,0 +1,324 @@
+// Maxwell kernels, B/D form with divergence cleaning (reference
+// src/physics/maxwell.py).
+//
+// maxwell_res_kernel: residuals rB = div B and rD = div D - rho on interior
+// cells (maxwell.py:491-508), with the edge padding of maxwell.py:511-512
+// written into the face ghosts.
+// maxwell_stage_kernel: one thread per interior cell; MUSCL/PC states of the
+// six fields on both faces along each direction, curl flux of H = B/mu and
+// E = D/eps (maxwell.py:137-181), Rusanov or HLL with the light-speed wave
+// set, geometric source (:366-371), Ohm current (:415-420), cleaning source
+// (:514-528), divergence and the fused stage update.
+#include "common.h"
+
+namespace mfvm {
+
+struct MXPar {
+  double mu, ep, inv_mu, inv_ep;  // inverses used when mu / eps are powers of two
+  int mu_pow2, ep_pow2;
+  double c;                        // 1/sqrt(mu*eps)
+  double negsigma, rho, negchiB, negchiD;
+};
+
+struct MXArgs {
+  Layout lay;
+  ILayout il;
+  Geo geo;
+  MXPar p;
+  const void* U;
+  const void* Rr;  // residuals [2][ghosted]
+  StageIO io;
+};
+
+template <class R>
+MFVM_DEV R mx_H(const MXPar& p, R b) { return p.mu_pow2 ? b * R(p.inv_mu) : b / R(p.mu); }
+template <class R>
+MFVM_DEV R mx_E(const MXPar& p, R d) { return p.ep_pow2 ? d * R(p.inv_ep) : d / R(p.ep); }
+
+// ---------------------------------------------------------------------------
+// Residuals
+// ---------------------------------------------------------------------------
+template <class R>
+MFVM_DEV R mx_div(const Layout& lay, const Geo& g, const R* F, int i, int j, int k) {
+  const long long o = lay.at(i, j, k), fs = lay.fs;
+  const long long st[3] = {1, lay.sy, lay.sz};
+  R gr[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const R d = F[o + a * fs + st[a]] - F[o + a * fs - st[a]];
+    gr[a] = g.lpow2[a] ? d * R(0.5 * g.linv[a]) : d / (R(2) * met<R>(g, MFVM_MET_L1 + a, i - 1, j - 1, k - 1));
+  }
+  R div = (gr[0] + gr[1]) + gr[2];
+  if (g.curv) {
+    const int a = i - 1, b = j - 1, e = k - 1;
+    const R c1 = F[o], c2 = F[o + fs], c3 = F[o + 2 * fs];
+    div = ((div + (c1 * (met<R>(g, MFVM_MET_D1H2, a, b, e) + met<R>(g, MFVM_MET_D1H3, a, b, e))) /
+                      met<R>(g, MFVM_MET_H1, a, b, e)) +
+           (c2 * (met<R>(g, MFVM_MET_D2H1, a, b, e) + met<R>(g, MFVM_MET_D2H3, a, b, e))) /
+               met<R>(g, MFVM_MET_H2, a, b, e)) +
+          (c3 * (met<R>(g, MFVM_MET_D3H1, a, b, e) + met<R>(g, MFVM_MET_D3H2, a, b, e))) /
+              met<R>(g, MFVM_MET_H3, a, b, e);
+  }
+  return div;
+}
+
+template <class R>
+__global__ void maxwell_res_kernel(Layout lay, Geo geo, MXPar p, const R* __restrict__ U, R* __restrict__ Rr) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  const int n1 = lay.n1, n2 = lay.n2, n3 = lay.n3;
+  if (i > n1 + 1 || j > n2 + 1) return;
+  const int nb = (i == 0 || i == n1 + 1) + (j == 0 || j == n2 + 1) + (k == 0 || k == n3 + 1);
+  if (nb > 1) return;
+  // edge padding: a face ghost takes the value of its adjacent interior cell
+  const int ii = i == 0 ? 1 : (i == n1 + 1 ? n1 : i);
+  const int jj = j == 0 ? 1 : (j == n2 + 1 ? n2 : j);
+  const int kk = k == 0 ? 1 : (k == n3 + 1 ? n3 : k);
+  const R rB = mx_div<R>(lay, geo, U, ii, jj, kk);
+  const R rD = mx_div<R>(lay, geo, U + 3 * lay.fs, ii, jj, kk) - R(p.rho);
+  const long long o = lay.at(i, j, k);
+  Rr[o] = rB;
+  Rr[o + lay.fs] = rD;
+}
+
+// ---------------------------------------------------------------------------
+// Stage
+// ---------------------------------------------------------------------------
+
+// Curl flux along D of one state (B[3], D[3]) -> F[6].
+template <int D, class R>
+MFVM_DEV void mx_flux(const MXPar& p, const R* u, R* F) {
+  const R H1 = mx_H<R>(p, u[0]), H2 = mx_H<R>(p, u[1]), H3 = mx_H<R>(p, u[2]);
+  const R E1 = mx_E<R>(p, u[3]), E2 = mx_E<R>(p, u[4]), E3 = mx_E<R>(p, u[5]);
+  if (D == 0) {
+    F[0] = R(0); F[1] = -E3; F[2] = E2;
+    F[3] = R(0); F[4] = H3; F[5] = -H2;
+  } else if (D == 1) {
+    F[0] = E3; F[1] = R(0); F[2] = -E1;
+    F[3] = -H3; F[4] = R(0); F[5] = H1;
+  } else {
+    F[0] = -E2; F[1] = E1; F[2] = R(0);
+    F[3] = H2; F[4] = -H1; F[5] = R(0);
+  }
+}
+
+template <int D, int RS, class R>
+MFVM_DEV void mx_face(const MXPar& p, const R* l, const R* r, R area, R* fs) {
+  R fl[6], fr[6];
+  mx_flux<D, R>(p, l, fl);
+  mx_flux<D, R>(p, r, fr);
+  const R c = R(p.c);
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    R F;
+    if (RS == MFVM_RS_RUSANOV) {
+      F = ((fl[q] + fr[q]) / R(2)) - ((c * (r[q] - l[q])) / R(2));
+    } else {
+      const R smin = -c, smax = c;
+      F = ((smax * fl[q] - smin * fr[q]) + (smin * smax) * (r[q] - l[q])) / (smax - smin);
+    }
+    fs[q] = (F + R(0)) * area;
+  }
+}
+
+template <int D, int LIM, class R>
+MFVM_DEV void mx_states(const MXArgs& a, const R* U, int i, int j, int k, R lo[6], R hi[6]) {
+  const long long fs = a.lay.fs;
+  const long long st = D == 0 ? 1 : (D == 1 ? a.lay.sy : a.lay.sz);
+  const long long o = a.lay.at(i, j, k);
+  R q[6];
+#pragma unroll
+  for (int f = 0; f < 6; ++f) q[f] = U[o + f * fs];
+  const int c = D == 0 ? i : (D == 1 ? j : k);
+  const int n = D == 0 ? a.lay.n1 : (D == 1 ? a.lay.n2 : a.lay.n3);
+  if (LIM == MFVM_LIM_NONE || c < 1 || c > n) {
+#pragma unroll
+    for (int f = 0; f < 6; ++f) lo[f] = hi[f] = q[f];
+    return;
+  }
+  const R len = a.geo.lpow2[D] ? R(1.0 / a.geo.linv[D]) : met<R>(a.geo, MFVM_MET_L1 + D, i - 1, j - 1, k - 1);
+#pragma unroll
+  for (int f = 0; f < 6; ++f) {
+    const R qm = U[o + f * fs - st], qp = U[o + f * fs + st];
+    const R b = div_len<R>(a.geo, D, q[f] - qm, len);
+    const R w = div_len<R>(a.geo, D, qp - q[f], len);
+    const R s = limit<LIM, R>(b, w) * len;
+    hi[f] = q[f] + s / R(2);
+    lo[f] = q[f] - s / R(2);
+  }
+}
+
+template <int D, int LIM, int RS, class R>
+MFVM_DEV void mx_dir(const MXArgs& a, const R* U, int i, int j, int k, R* acc) {
+  const int di = D == 0, dj = D == 1, dk = D == 2;
+  R lo_m[6], hi_m[6], lo_c[6], hi_c[6], lo_p[6], hi_p[6];
+  mx_states<D, LIM, R>(a, U, i - di, j - dj, k - dk, lo_m, hi_m);
+  mx_states<D, LIM, R>(a, U, i, j, k, lo_c, hi_c);
+  mx_states<D, LIM, R>(a, U, i + di, j + dj, k + dk, lo_p, hi_p);
+  const int m = MFVM_MET_S1 + D;
+  const R s_lo = met<R>(a.geo, m, i - 1, j - 1, k - 1);
+  const R s_hi = met<R>(a.geo, m, i - 1 + di, j - 1 + dj, k - 1 + dk);
+  R f_lo[6], f_hi[6];
+  mx_face<D, RS, R>(a.p, hi_m, lo_c, s_lo, f_lo);
+  mx_face<D, RS, R>(a.p, hi_c, lo_p, s_hi, f_hi);
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    if (D == 0) acc[q] = f_hi[q] - f_lo[q];
+    else acc[q] = (acc[q] + f_hi[q]) - f_lo[q];
+  }
+}
+
+template <int LIM, int RS, class R>
+__global__ __launch_bounds__(256) void maxwell_stage_kernel(MXArgs a) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > a.lay.n1 || j > a.lay.n2) return;
+  const R* U = static_cast<const R*>(a.U);
+  const R* Rr = static_cast<const R*>(a.Rr);
+  R acc[6];
+  mx_dir<0, LIM, RS, R>(a, U, i, j, k, acc);
+  mx_dir<1, LIM, RS, R>(a, U, i, j, k, acc);
+  mx_dir<2, LIM, RS, R>(a, U, i, j, k, acc);
+
+  const long long o = a.lay.at(i, j, k), fs = a.lay.fs;
+  R u[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) u[q] = U[o + q * fs];
+  // geometric source (exactly +-0 on Cartesian grids)
+  R sg[6] = {R(0), R(0), R(0), R(0), R(0), R(0)};
+  if (a.geo.curv) {
+    const int x = i - 1, y = j - 1, z = k - 1;
+    const R h1 = met<R>(a.geo, MFVM_MET_H1, x, y, z), h2 = met<R>(a.geo, MFVM_MET_H2, x, y, z),
+            h3 = met<R>(a.geo, MFVM_MET_H3, x, y, z);
+    const R d1h2 = met<R>(a.geo, MFVM_MET_D1H2, x, y, z), d1h3 = met<R>(a.geo, MFVM_MET_D1H3, x, y, z);
+    const R d2h1 = met<R>(a.geo, MFVM_MET_D2H1, x, y, z), d2h3 = met<R>(a.geo, MFVM_MET_D2H3, x, y, z);
+    const R d3h1 = met<R>(a.geo, MFVM_MET_D3H1, x, y, z), d3h2 = met<R>(a.geo, MFVM_MET_D3H2, x, y, z);
+    const R H1 = mx_H<R>(a.p, u[0]), H2 = mx_H<R>(a.p, u[1]), H3 = mx_H<R>(a.p, u[2]);
+    const R E1 = mx_E<R>(a.p, u[3]), E2 = mx_E<R>(a.p, u[4]), E3 = mx_E<R>(a.p, u[5]);
+    // tensors t[a][b] with 0.0 on the diagonal (maxwell.py:146-165)
+    const R tb[3][3] = {{R(0), E3, -E2}, {-E3, R(0), E1}, {E2, -E1, R(0)}};
+    const R td[3][3] = {{R(0), -H3, H2}, {H3, R(0), -H1}, {-H2, H1, R(0)}};
+#pragma unroll
+    for (int w = 0; w < 2; ++w) {
+      const R(*t)[3] = w == 0 ? tb : td;
+      sg[3 * w + 0] = ((((t[0][1] * d2h1) / h2) + ((t[0][2] * d3h1) / h3)) - ((t[1][1] * d1h2) / h1)) -
+                      ((t[2][2] * d1h3) / h1);
+      sg[3 * w + 1] = ((((t[1][0] * d1h2) / h1) + ((t[1][2] * d3h2) / h3)) - ((t[0][0] * d2h1) / h2)) -
+                      ((t[2][2] * d2h3) / h2);
+      sg[3 * w + 2] = ((((t[2][0] * d1h3) / h1) + ((t[2][1] * d2h3) / h2)) - ((t[0][0] * d3h1) / h3)) -
+                      ((t[1][1] * d3h2) / h3);
+    }
+  }
+  // cleaning source: -chi * centred gradient of the edge-padded residual
+  const long long st[3] = {1, a.lay.sy, a.lay.sz};
+  R sc[6];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const R len2 = a.geo.lpow2[d] ? R(0) : R(2) * met<R>(a.geo, MFVM_MET_L1 + d, i - 1, j - 1, k - 1);
+#pragma unroll
+    for (int w = 0; w < 2; ++w) {
+      const R dr = Rr[o + w * fs + st[d]] - Rr[o + w * fs - st[d]];
+      const R gr = a.geo.lpow2[d] ? dr * R(0.5 * a.geo.linv[d]) : dr / len2;
+      sc[3 * w + d] = R(w == 0 ? a.p.negchiB : a.p.negchiD) * gr;
+    }
+  }
+  R kr[6];
+  R vol = a.geo.vpow2 ? R(0) : met<R>(a.geo, MFVM_MET_V, i - 1, j - 1, k - 1);
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    const R sp = q < 3 ? R(0) : R(a.p.negsigma) * mx_E<R>(a.p, u[q]);
+    const R src = (sg[q] + sp) + sc[q];
+    const R x = -acc[q];
+    kr[q] = (a.geo.vpow2 ? x * R(a.geo.vinv) : x / vol) + src;
+  }
+  stage_store<R>(a.io, 6, fs, a.il.fs, o, a.il.at(i, j, k), u, kr, nullptr);
+}
+
+// dt = cfl / (max_d c/Lmin_d + (2 chi_max) smax)   (maxwell.py:579-598)
+__global__ void maxwell_dt_kernel(double c, double l1, double l2, double l3, double chimax, double smax,
+                                  double cfl, double* dt_out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    const double rc = nmax(nmax(c / l1, c / l2), c / l3);
+    *dt_out = cfl / (rc + (2.0 * chimax) * smax);
+  }
+}
+
+static MXPar make_mx(const mfvm_maxwell_params* q) {
+  MXPar p;
+  int e;
+  p.mu = q->mu;
+  p.ep = q->ep;
+  p.mu_pow2 = frexp(q->mu, &e) == 0.5;
+  p.ep_pow2 = frexp(q->ep, &e) == 0.5;
+  p.inv_mu = p.mu_pow2 ? 1.0 / q->mu : 0.0;
+  p.inv_ep = p.ep_pow2 ? 1.0 / q->ep : 0.0;
+  p.c = 1.0 / sqrt(q->mu * q->ep);
+  p.negsigma = -q->sigma;
+  p.rho = q->rho;
+  p.negchiB = -q->chi_b;
+  p.negchiD = -q->chi_d;
+  return p;
+}
+
+template <class R>
+hipError_t maxwell_launch_res(const mfvm_grid* g, const mfvm_maxwell_params* q, const void* U, void* Rr,
+                              hipStream_t s) {
+  const Layout lay = make_layout(g->n[0], g->n[1], g->n[2]);
+  const dim3 blk(64, 4, 1);
+  const dim3 grd((g->n[0] + 2 + 63) / 64, (g->n[1] + 2 + 3) / 4, g->n[2] + 2);
+  const int tk = t_begin(MFVM_TK_CLOSURE, s);
+  hipLaunchKernelGGL(maxwell_res_kernel<R>, grd, blk, 0, s, lay, make_geo(g), make_mx(q),
+                     static_cast<const R*>(U), static_cast<R*>(Rr));
+  t_end(tk, s);
+  return hipGetLastError();
+}
+
+template <int LIM, int RS, class R>
+static hipError_t mx_launch_t(const MXArgs& a, hipStream_t s) {
+  const dim3 blk(64, 4, 1);
+  const dim3 grd((a.lay.n1 + 63) / 64, (a.lay.n2 + 3) / 4, a.lay.n3);
+  const int tk = t_begin(MFVM_TK_STAGE, s);
+  hipLaunchKernelGGL((maxwell_stage_kernel<LIM, RS, R>), grd, blk, 0, s, a);
+  t_end(tk, s);
+  return hipGetLastError();
+}
+
+template <class R>
+hipError_t maxwell_launch_stage(const mfvm_grid* g, const mfvm_maxwell_params* q, const void* U,
+                                const void* Rr, const StageIO& io, hipStream_t s) {
+  MXArgs a;
+  a.lay = make_layout(g->n[0], g->n[1], g->n[2]);
+  a.il = make_ilayout(g->n[0], g->n[1], g->n[2]);
+  a.geo = make_geo(g);
+  a.p = make_mx(q);
+  a.U = U;
+  a.Rr = Rr;
+  a.io = io;
+#define MX_RS(L)                                                                          \
+  return q->riemann == MFVM_RS_HLL ? mx_launch_t<L, MFVM_RS_HLL, R>(a, s) : mx_launch_t<L, MFVM_RS_RUSANOV, R>(a, s);
+  switch (q->limiter) {
+    case MFVM_LIM_NONE: MX_RS(MFVM_LIM_NONE)
+    case MFVM_LIM_MINMOD: MX_RS(MFVM_LIM_MINMOD)
+    case MFVM_LIM_MC: MX_RS(MFVM_LIM_MC)
+    default: MX_RS(MFVM_LIM_VANLEER)
+  }
+#undef MX_RS
+}
+
+hipError_t maxwell_launch_dt(const mfvm_grid* g, const mfvm_maxwell_params* q, double cfl, double* dt_out,
+                             hipStream_t s) {
+  const double c = 1.0 / sqrt(q->mu * q->ep);
+  const double chimax = q->chi_b > q->chi_d ? q->chi_b : q->chi_d;  // Python max(chi_B, chi_D)
+  hipLaunchKernelGGL(maxwell_dt_kernel, dim3(1), dim3(64), 0, s, c, g->lmin[0], g->lmin[1], g->lmin[2], chimax,
+                     g->smax, cfl, dt_out);
+  return hipGetLastError();
+}
+
+template hipError_t maxwell_launch_res<double>(const mfvm_grid*, const mfvm_maxwell_params*, const void*, void*, hipStream_t);
+template hipError_t maxwell_launch_res<float>(const mfvm_grid*, const mfvm_maxwell_params*, const void*, void*, hipStream_t);
+template hipError_t maxwell_launch_stage<double>(const mfvm_grid*, const mfvm_maxwell_params*, const void*, const void*, const StageIO&, hipStream_t);
+template hipError_t maxwell_launch_stage<float>(const mfvm_grid*, const mfvm_maxwell_params*, const void*, const void*, const StageIO&, hipStream_t);
+
+}  // namespace mfvm

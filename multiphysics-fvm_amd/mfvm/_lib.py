@@ -31,7 +31,7 @@ class Grid(ctypes.Structure):
                 ("fac", ctypes.c_void_p), ("off", (ctypes.c_int32 * 3) * NMET),
                 ("lpow2", ctypes.c_int32 * 3), ("vpow2", ctypes.c_int32),
                 ("lval", ctypes.c_double * 3), ("vval", ctypes.c_double),
-                ("smax", ctypes.c_double)]
+                ("smax", ctypes.c_double), ("lmin", ctypes.c_double * 3)]
 
 
 class NSParams(ctypes.Structure):
@@ -48,6 +48,19 @@ class Stage(ctypes.Structure):
                 ("c0", ctypes.c_double), ("c1", ctypes.c_double),
                 ("acc_coef", ctypes.c_double), ("dt", ctypes.c_double),
                 ("dt_dev", ctypes.c_void_p), ("partials", ctypes.c_void_p)]
+
+
+class HeatParams(ctypes.Structure):
+    _fields_ = [("dtype", ctypes.c_int32), ("limiter", ctypes.c_int32),
+                ("bc", (ctypes.c_int32 * 6) * 1), ("rho_cp", ctypes.c_double),
+                ("conductivity", ctypes.c_double), ("source", ctypes.c_double)]
+
+
+class MaxwellParams(ctypes.Structure):
+    _fields_ = [("dtype", ctypes.c_int32), ("limiter", ctypes.c_int32),
+                ("riemann", ctypes.c_int32), ("bc", (ctypes.c_int32 * 6) * 6),
+                ("mu", ctypes.c_double), ("ep", ctypes.c_double), ("sigma", ctypes.c_double),
+                ("rho", ctypes.c_double), ("chi_b", ctypes.c_double), ("chi_d", ctypes.c_double)]
 
 
 class NSWork(ctypes.Structure):
@@ -75,6 +88,20 @@ SIGNATURES = [
     ("mfvm_ns_advance", ctypes.c_int, [_G, _NP, ctypes.POINTER(NSWork), ctypes.c_int32,
                                        ctypes.c_double, ctypes.c_int32, _P]),
     ("mfvm_ns_partials_size", ctypes.c_int64, [_G, _NP]),
+    ("mfvm_heat_rhs", ctypes.c_int, [_G, ctypes.POINTER(HeatParams), _P, _P, _P]),
+    ("mfvm_heat_dt", ctypes.c_int, [_G, ctypes.POINTER(HeatParams), ctypes.c_double, _P, _P]),
+    ("mfvm_heat_step", ctypes.c_int, [_G, ctypes.POINTER(HeatParams), ctypes.POINTER(NSWork),
+                                      ctypes.c_int32, ctypes.c_double, _P]),
+    ("mfvm_heat_advance", ctypes.c_int, [_G, ctypes.POINTER(HeatParams), ctypes.POINTER(NSWork),
+                                         ctypes.c_int32, ctypes.c_double, ctypes.c_int32, _P]),
+    ("mfvm_maxwell_residual", ctypes.c_int, [_G, ctypes.POINTER(MaxwellParams), _P, _P, _P]),
+    ("mfvm_maxwell_rhs", ctypes.c_int, [_G, ctypes.POINTER(MaxwellParams), _P, _P, _P, _P]),
+    ("mfvm_maxwell_dt", ctypes.c_int, [_G, ctypes.POINTER(MaxwellParams), ctypes.c_double, _P, _P]),
+    ("mfvm_maxwell_step", ctypes.c_int, [_G, ctypes.POINTER(MaxwellParams), ctypes.POINTER(NSWork),
+                                         ctypes.c_int32, ctypes.c_double, _P]),
+    ("mfvm_maxwell_advance", ctypes.c_int, [_G, ctypes.POINTER(MaxwellParams),
+                                            ctypes.POINTER(NSWork), ctypes.c_int32,
+                                            ctypes.c_double, ctypes.c_int32, _P]),
 ]
 
 

@@ -153,6 +153,9 @@ class GridDesc:
             d.lval[a] = val
         d.vpow2, d.vval = _uniform_pow2(*F[_lib.MET_V])
         d.smax = self._smax(F)
+        for a in range(3):
+            fa, fb, fc = F[_lib.MET_L1 + a]
+            d.lmin[a] = float(np.min((fa[:, None, None] * fb[None, :, None]) * fc[None, None, :]))
         self.desc = d
 
     @staticmethod
