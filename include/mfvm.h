@@ -60,8 +60,8 @@ extern "C" {
 #define MFVM_F32 1
 
 /* Metric slots.  Metric m at cell/face (i,j,k) is (a[i]*b[j])*c[k] with
- * a = fac + off[m][0], b = fac + off[m][1], c = fac + off[m][2] (0-based
- * interior or face indices).  This reproduces geometry.build_grid
+ * a = fac + (3m)*fstride, b = fac + (3m+1)*fstride, c = fac + (3m+2)*fstride
+ * (0-based interior or face indices).  This reproduces geometry.build_grid
  * (src/grid/geometry.py:80-171) value for value. */
 #define MFVM_MET_V 0
 #define MFVM_MET_S1 1
@@ -88,12 +88,14 @@ extern "C" {
 typedef struct mfvm_grid {
   int32_t n[3];                 /* interior cells nx1, nx2, nx3 */
   int32_t coord;                /* MFVM_CART / MFVM_CYL / MFVM_SPH */
-  const double* fac;            /* device: packed 1D metric factors */
-  int32_t off[MFVM_NMET][3];
+  const double* fac;            /* device: 1D metric factors, 3*MFVM_NMET rows of fstride */
+  int32_t fstride;              /* >= max(nx1, nx2, nx3) + 1 */
+  int32_t uniform;              /* Cartesian with one L_d per direction: L, V, S constant */
   int32_t lpow2[3];             /* 1 if L_d is the same power of two in every cell */
   int32_t vpow2;                /* 1 if V is the same power of two in every cell */
-  double lval[3];               /* that L_d (when lpow2[d]) */
-  double vval;                  /* that V (when vpow2) */
+  double lval[3];               /* that L_d (when lpow2[d] or uniform) */
+  double vval;                  /* that V (when vpow2 or uniform) */
+  double sval[3];               /* the face areas S_d (when uniform) */
   double smax;                  /* max over cells of 1/L1^2 + 1/L2^2 + 1/L3^2 */
   double lmin[3];               /* min over cells of L_d */
 } mfvm_grid;

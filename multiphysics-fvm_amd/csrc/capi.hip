@@ -80,18 +80,17 @@ static int hip_check(hipError_t e, const char* where) {
 Geo make_geo(const mfvm_grid* g) {
   Geo o;
   o.fac = g->fac;
-  for (int m = 0; m < MFVM_NMET; ++m)
-    for (int d = 0; d < 3; ++d) o.off[m][d] = g->off[m][d];
-  o.n1 = g->n[0];
-  o.n2 = g->n[1];
-  o.n3 = g->n[2];
+  o.fstride = g->fstride;
   o.curv = g->coord != MFVM_CART;
   for (int d = 0; d < 3; ++d) {
     o.lpow2[d] = g->lpow2[d];
     o.linv[d] = g->lpow2[d] ? 1.0 / g->lval[d] : 0.0;
+    o.lval[d] = g->lval[d];
+    o.sval[d] = g->sval[d];
   }
   o.vpow2 = g->vpow2;
   o.vinv = g->vpow2 ? 1.0 / g->vval : 0.0;
+  o.vval = g->vval;
   return o;
 }
 
@@ -103,6 +102,9 @@ static int check_grid(const mfvm_grid* g) {
     return fail(MFVM_E_ARG,
                 "Unknown coordinate system. Expected one of: cartesian, cylindrical, spherical.");
   if (!g->fac) return fail(MFVM_E_ARG, "grid metric factors are NULL");
+  const int mx = g->n[0] > g->n[1] ? (g->n[0] > g->n[2] ? g->n[0] : g->n[2]) : (g->n[1] > g->n[2] ? g->n[1] : g->n[2]);
+  if (g->fstride < mx + 1) return fail(MFVM_E_ARG, "grid factor stride must be >= max(n)+1");
+  if (g->uniform && g->coord != MFVM_CART) return fail(MFVM_E_ARG, "uniform grids must be Cartesian");
   return MFVM_OK;
 }
 

@@ -29,45 +29,68 @@ template <class R> MFVM_DEV R nmin(R a, R b) { return (a < b || a != a) ? a : b;
 template <class R> MFVM_DEV R nmax(R a, R b) { return (a > b || a != a) ? a : b; }
 
 // Limited slope, reference slope_limiters.py.  LIM: 1 minmod, 2 MC, 3 van Leer.
+//
+// The reference multiplies a sign-agreement factor by a magnitude.  When both
+// slopes are nonzero with one sign the factor is exactly +-1, so the product
+// is the magnitude with the sign of b (copysign: bit-identical).  In every
+// other case the reference value is a zero (or NaN when an input is NaN),
+// reproduced by (b + f) * 0 up to the sign of that zero.  MC's 2-to-1 split
+// keeps the reference's +-1/3 factor (slope_limiters.py:66-71).
 template <int LIM, class R> MFVM_DEV R limit(R b, R f) {
+  const bool same = ((b > R(0)) & (f > R(0))) | ((b < R(0)) & (f < R(0)));
+  const R zero = (b + f) * R(0);
   if constexpr (LIM == MFVM_LIM_MINMOD) {
     // slope_limiters.py:30-37
-    R agree = (nsign(b) + nsign(f)) / R(2);
-    R least = nmin(fabs(b), fabs(f));
-    return agree * least;
+    const R least = nmin(fabs(b), fabs(f));
+    return same ? copysign(least, b) : zero;
   } else if constexpr (LIM == MFVM_LIM_MC) {
     // slope_limiters.py:60-71 (sign sum over three candidates divided by 3)
-    R c = R(0.5) * (b + f);
-    R bb = R(2) * b;
-    R ff = R(2) * f;
-    R agree = ((nsign(c) + nsign(bb)) + nsign(ff)) / R(3);
-    R least = nmin(fabs(c), nmin(fabs(bb), fabs(ff)));
-    return agree * least;
+    const R c = R(0.5) * (b + f);
+    const R least = nmin(fabs(c), nmin(fabs(R(2) * b), fabs(R(2) * f)));
+    const bool opp = ((b > R(0)) & (f < R(0))) | ((b < R(0)) & (f > R(0)));
+    const R third = copysign(R(1.0 / 3.0) * least, c);
+    return same ? copysign(least, b) : (opp ? third : zero);
   } else {
     // slope_limiters.py:94-103 (harmonic mean on denom > 0, else 0)
-    R agree = (nsign(b) + nsign(f)) / R(2);
-    R denom = fabs(b) + fabs(f);
-    R harm = denom > R(0) ? ((R(2) * b) * f) / denom : R(0);
-    return agree * harm;
+    const R harm = ((R(2) * b) * f) / (fabs(b) + fabs(f));
+    return same ? copysign(harm, b) : zero;
   }
 }
 
-// Geometry: every metric is M[i,j,k] = (a[i] * b[j]) * c[k] over 1D factors.
+// Geometry: every metric is M[i,j,k] = (a[i] * b[j]) * c[k] over 1D factors
+// stored as rows of a fixed stride; uniform Cartesian grids use constants.
 struct Geo {
   const double* fac;
-  int off[MFVM_NMET][3];
-  int n1, n2, n3;
+  int fstride;
   int curv;             // 0 cartesian (Lame terms vanish), 1 curvilinear
   int lpow2[3];         // L_d is one power of two everywhere
-  double linv[3];       // its exact reciprocal
   int vpow2;
-  double vinv;
+  double linv[3];       // exact reciprocal of L_d when lpow2
+  double lval[3];       // L_d when uniform
+  double vinv, vval;
+  double sval[3];       // S_d when uniform
 };
 
 // i, j, k are 0-based interior (or face) indices of the metric's own array.
 template <class R> MFVM_DEV R met(const Geo& g, int m, int i, int j, int k) {
-  const double* f = g.fac;
-  return R((f[g.off[m][0] + i] * f[g.off[m][1] + j]) * f[g.off[m][2] + k]);
+  const double* f = g.fac + (long long)(3 * m) * g.fstride;
+  return R((f[i] * f[g.fstride + j]) * f[2 * g.fstride + k]);
+}
+
+// Cell length L_d, face area S_d and volume V; UNIF: constants of a uniform
+// Cartesian grid (exactly the reference's values), else factor products.
+// (i, j, k) are ghosted cell indices; face indices are 0-based face numbers.
+template <bool UNIF, class R> MFVM_DEV R g_len(const Geo& g, int d, int i, int j, int k) {
+  if (UNIF) return R(g.lval[d]);
+  return met<R>(g, MFVM_MET_L1 + d, i - 1, j - 1, k - 1);
+}
+template <bool UNIF, class R> MFVM_DEV R g_area(const Geo& g, int d, int fi, int fj, int fk) {
+  if (UNIF) return R(g.sval[d]);
+  return met<R>(g, MFVM_MET_S1 + d, fi, fj, fk);
+}
+template <bool UNIF, class R> MFVM_DEV R g_vol(const Geo& g, int i, int j, int k) {
+  if (UNIF) return R(g.vval);
+  return met<R>(g, MFVM_MET_V, i - 1, j - 1, k - 1);
 }
 
 // x / L_d at cell (i,j,k) (0-based interior): exact reciprocal multiply when

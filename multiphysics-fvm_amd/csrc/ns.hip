@@ -57,7 +57,7 @@ struct StageArgs {
 // ---------------------------------------------------------------------------
 // Closure (navier_stokes.py:44-156): P on interior + face ghosts, T interior.
 // ---------------------------------------------------------------------------
-template <class R>
+template <bool UNIF, class R>
 __global__ void ns_closure_kernel(Layout lay, Geo geo, NSPar p, const R* __restrict__ U,
                                   R* __restrict__ C) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -90,7 +90,7 @@ __global__ void ns_closure_kernel(Layout lay, Geo geo, NSPar p, const R* __restr
 #pragma unroll
       for (int a = 0; a < 3; ++a) g[a][b] = (vp[a] - vm[a]) * R(0.5 * geo.linv[b]);
     } else {
-      twoL = R(2) * met<R>(geo, MFVM_MET_L1 + b, i - 1, j - 1, k - 1);
+      twoL = R(2) * g_len<UNIF, R>(geo, b, i, j, k);
 #pragma unroll
       for (int a = 0; a < 3; ++a) g[a][b] = (vp[a] - vm[a]) / twoL;
     }
@@ -98,7 +98,7 @@ __global__ void ns_closure_kernel(Layout lay, Geo geo, NSPar p, const R* __restr
   const R mu = R(p.mu);
   const R two3 = R(2.0 / 3.0);
   R t11, t22, t33, t12, t13, t23;
-  if (!geo.curv) {
+  if (UNIF || !geo.curv) {
     // Cartesian: every Lame term is exactly +-0 and drops out of the sums.
     const R divv = (g[0][0] + g[1][1]) + g[2][2];
     t11 = mu * ((R(2) * g[0][0]) - (two3 * divv));
@@ -154,18 +154,18 @@ MFVM_DEV void load_q(const R* __restrict__ U, const R* __restrict__ C, long long
 }
 
 // Cell length along D at ghosted cell (i,j,k) (interior only).
-template <int D, class R>
+template <int D, bool UNIF, class R>
 MFVM_DEV R cell_len(const Geo& g, int i, int j, int k) {
-  if (g.lpow2[D]) return R(1.0 / g.linv[D]);
-  return met<R>(g, MFVM_MET_L1 + D, i - 1, j - 1, k - 1);
+  if (g.lpow2[D]) return R(g.lval[D]);
+  return g_len<UNIF, R>(g, D, i, j, k);
 }
 
 // reconstructors.py:115-148 for one cell: limited slope from (qm, q, qp),
 // face states q + s/2 (left side of the cell's upper face) and q - s/2.
-template <int D, int LIM, class R>
+template <int D, int LIM, bool UNIF, class R>
 MFVM_DEV void slope_states(const Geo& g, int i, int j, int k, int nf, const R qm[NQ],
                            const R q[NQ], const R qp[NQ], R lo[NQ], R hi[NQ]) {
-  const R len = cell_len<D, R>(g, i, j, k);
+  const R len = cell_len<D, UNIF, R>(g, i, j, k);
 #pragma unroll
   for (int f = 0; f < NQ; ++f) {
     if (f < nf) {
@@ -184,7 +184,7 @@ MFVM_DEV void slope_states(const Geo& g, int i, int j, int k, int nf, const R qm
 // States of ghosted cell (i,j,k) along D, loading the stencil from memory.
 // hi: state on the cell's upper face (reference fields_L), lo: on its lower
 // face (fields_R).  Ghost cells carry zero slope (reconstructors.py:127-132).
-template <int D, int LIM, class R>
+template <int D, int LIM, bool UNIF, class R>
 MFVM_DEV void cell_states(const StageArgs& a, const R* U, const R* C, int i, int j, int k,
                           R lo[NQ], R hi[NQ]) {
   const long long fs = a.lay.fs;
@@ -202,7 +202,7 @@ MFVM_DEV void cell_states(const StageArgs& a, const R* U, const R* C, int i, int
   R qm[NQ], qp[NQ];
   load_q<D, R>(U, C, fs, o - st, a.p.visc, qm);
   load_q<D, R>(U, C, fs, o + st, a.p.visc, qp);
-  slope_states<D, LIM, R>(a.geo, i, j, k, a.p.visc ? 9 : 6, qm, q, qp, lo, hi);
+  slope_states<D, LIM, UNIF, R>(a.geo, i, j, k, a.p.visc ? 9 : 6, qm, q, qp, lo, hi);
 }
 
 // Normal flux F.S through a D-face from left state l and right state r:
@@ -270,18 +270,18 @@ MFVM_DEV void face_flux(const NSPar& p, const R l[NQ], const R r[NQ], R area, R 
 
 // Source terms at interior cell (i,j,k) (navier_stokes.py:358-509), given
 // the 12 stored quantities of the cell: U(5), P, T11,T22,T33,T12,T13,T23.
-template <class R>
+template <bool UNIF, class R>
 MFVM_DEV void cell_source(const Geo& g, const NSPar& p, int i, int j, int k, const R* u,
                           const R* c, R src[5], bool& any) {
   any = false;
 #pragma unroll
   for (int q = 0; q < 5; ++q) src[q] = R(0);
-  if (!g.curv && !p.grav) return;
+  if ((UNIF || !g.curv) && !p.grav) return;
   any = true;
   const R rho = u[0];
   const R v1 = u[1] / rho, v2 = u[2] / rho, v3 = u[3] / rho;
   R sg1 = R(0), sg2 = R(0), sg3 = R(0);
-  if (g.curv) {
+  if (!UNIF && g.curv) {
     const int a = i - 1, b = j - 1, e = k - 1;
     const R h1 = met<R>(g, MFVM_MET_H1, a, b, e), h2 = met<R>(g, MFVM_MET_H2, a, b, e),
             h3 = met<R>(g, MFVM_MET_H3, a, b, e);
@@ -322,7 +322,7 @@ MFVM_DEV void cell_source(const Geo& g, const NSPar& p, int i, int j, int k, con
 }
 
 // CFL rates of one cell (navier_stokes.py:549-575): max_d (|v_d|+cs)/L_d and mu/rho.
-template <class R>
+template <bool UNIF, class R>
 MFVM_DEV void cell_cfl(const Geo& g, const NSPar& p, int i, int j, int k, const R u[5], R& rate,
                        R& nu) {
   const R rho = u[0];
@@ -331,9 +331,9 @@ MFVM_DEV void cell_cfl(const Geo& g, const NSPar& p, int i, int j, int k, const 
   const R P = (R(p.gm1) * rho) * (E - (((v1 * v1) + (v2 * v2)) + (v3 * v3)) / R(2));
   const R cs = sqrt((R(p.gamma) * P) / rho);
   const R a1 = fabs(v1) + cs, a2 = fabs(v2) + cs, a3 = fabs(v3) + cs;
-  const R r1 = div_len<R>(g, 0, a1, cell_len<0, R>(g, i, j, k));
-  const R r2 = div_len<R>(g, 1, a2, cell_len<1, R>(g, i, j, k));
-  const R r3 = div_len<R>(g, 2, a3, cell_len<2, R>(g, i, j, k));
+  const R r1 = div_len<R>(g, 0, a1, cell_len<0, UNIF, R>(g, i, j, k));
+  const R r2 = div_len<R>(g, 1, a2, cell_len<1, UNIF, R>(g, i, j, k));
+  const R r3 = div_len<R>(g, 2, a3, cell_len<2, UNIF, R>(g, i, j, k));
   rate = nmax(nmax(r1, r2), r3);
   nu = R(p.mu) / rho;
 }
@@ -369,7 +369,7 @@ MFVM_DEV void block_max2(R rate, R nu, double* partials, double* sred) {
 // ---------------------------------------------------------------------------
 // Fused stage kernel
 // ---------------------------------------------------------------------------
-template <int LIM, int RS, class R>
+template <int LIM, int RS, bool UNIF, class R>
 __global__ __launch_bounds__(256) void ns_stage_kernel(StageArgs a) {
   extern __shared__ double smem_raw[];
   R* sm = reinterpret_cast<R*>(smem_raw);
@@ -387,17 +387,18 @@ __global__ __launch_bounds__(256) void ns_stage_kernel(StageArgs a) {
   const bool own = (i <= n1) && (j <= n2);
   const bool visc = a.p.visc != 0;
 
-  // LDS carve-up (elements of R), shared by the x1 and x2 phases.
-  // x1: RX[9][BY][BX+1], LX[9][BY], FX[5][BY][BX+1]
-  // x2: RY[9][BY+1][BX], LY[9][BX], FY[5][BY+1][BX]
+  // LDS carve-up (elements of R): x1 and x2 exchange buffers side by side.
+  // RX[9][BY][BX+1] lower-face states, LX[9][BY] upper-face states of the
+  // x1 tile-edge cells, FX[5][BY][BX+1] face fluxes; RY/LY/FY likewise for x2.
   const int wx = BX + 1;
-  R* RX = sm;
-  R* LX = RX + NQ * BY * wx;
-  R* FX = LX + NQ * BY;
-  R* RY = sm;
-  R* LY = RY + NQ * (BY + 1) * BX;
-  R* FY = LY + NQ * BX;
   const int nrx = BY * wx, nry = (BY + 1) * BX;
+  R* RX = sm;
+  R* LX = RX + NQ * nrx;
+  R* FX = LX + NQ * BY;
+  R* RY = FX + 5 * nrx;
+  R* LY = RY + NQ * nry;
+  R* FY = LY + NQ * BX;
+  const int NT = BX * BY;
 
   double dt = a.dt_dev ? *a.dt_dev : a.dt;
 
@@ -438,7 +439,7 @@ __global__ __launch_bounds__(256) void ns_stage_kernel(StageArgs a) {
       load12(kb - 2, wm);
       zs(wm, zm);
       zs(cu, zc);
-      slope_states<2, LIM, R>(a.geo, i, j, kb - 1, visc ? 9 : 6, zm, zp, zc, lo, lz_prev);
+      slope_states<2, LIM, UNIF, R>(a.geo, i, j, kb - 1, visc ? 9 : 6, zm, zp, zc, lo, lz_prev);
     }
   }
 
@@ -456,18 +457,21 @@ __global__ __launch_bounds__(256) void ns_stage_kernel(StageArgs a) {
       } else {
         R zn[NQ];
         zs(nx, zn);
-        slope_states<2, LIM, R>(a.geo, i, j, m, visc ? 9 : 6, zp, zc, zn, lo, lz_cur);
+        slope_states<2, LIM, UNIF, R>(a.geo, i, j, m, visc ? 9 : 6, zp, zc, zn, lo, lz_cur);
       }
-      const R s3 = met<R>(a.geo, MFVM_MET_S3, i - 1, j - 1, m - 1);
+      const R s3 = g_area<UNIF, R>(a.geo, 2, i - 1, j - 1, m - 1);
       face_flux<2, RS, R>(a.p, lz_prev, lo, s3, fsz);
     }
 
     // ---- in-plane faces of plane m (all threads take part in the barriers) ----
+    // Phase 1: x1 and x2 states of the own cells and of the tile-edge cells;
+    // phase 2: F.S of every x1 and x2 face of the tile, each computed once;
+    // phase 3: flux differences.  Helper tasks go to the first threads in
+    // phase 1 and to the last threads in phase 2, so that no wave takes two.
     R accm[5], srcm[5];
     bool src_any = false;
     if (m < ke) {
-      // x1 phase
-      R hx[NQ];
+      R hx[NQ], hy[NQ];
       if (j <= n2 && i <= n1 + 1) {
         R lo[NQ];
         if (i <= n1) {
@@ -486,66 +490,14 @@ __global__ __launch_bounds__(256) void ns_stage_kernel(StageArgs a) {
             R qm[NQ], qp[NQ];
             load_q<0, R>(U, C, fs, o - 1, visc, qm);
             load_q<0, R>(U, C, fs, o + 1, visc, qp);
-            slope_states<0, LIM, R>(a.geo, i, j, m, visc ? 9 : 6, qm, q, qp, lo, hx);
+            slope_states<0, LIM, UNIF, R>(a.geo, i, j, m, visc ? 9 : 6, qm, q, qp, lo, hx);
           }
         } else {
-          cell_states<0, LIM, R>(a, U, C, i, j, m, lo, hx);  // x1 ghost: raw
+          cell_states<0, LIM, UNIF, R>(a, U, C, i, j, m, lo, hx);  // x1 ghost: raw
         }
 #pragma unroll
         for (int f = 0; f < NQ; ++f) RX[f * nrx + ty * wx + tx] = lo[f];
       }
-      // helpers: tile-edge cells in x1 (left: upper-face state, right: lower-face state)
-      for (int t = tid; t < 2 * BY; t += BX * BY) {
-        const int r = t % BY, side = t / BY;
-        const int jj = j0 + r;
-        if (jj <= n2) {
-          R lo[NQ], hi[NQ];
-          if (side == 0) {
-            cell_states<0, LIM, R>(a, U, C, i0 - 1, jj, m, lo, hi);
-#pragma unroll
-            for (int f = 0; f < NQ; ++f) LX[f * BY + r] = hi[f];
-          } else if (i0 + BX <= n1 + 1) {
-            cell_states<0, LIM, R>(a, U, C, i0 + BX, jj, m, lo, hi);
-#pragma unroll
-            for (int f = 0; f < NQ; ++f) RX[f * nrx + r * wx + BX] = lo[f];
-          }
-        }
-      }
-      __syncthreads();
-      if (own) {
-        R rr[NQ], fo[5];
-#pragma unroll
-        for (int f = 0; f < NQ; ++f) rr[f] = RX[f * nrx + ty * wx + tx + 1];
-        const R s1 = met<R>(a.geo, MFVM_MET_S1, i, j - 1, m - 1);
-        face_flux<0, RS, R>(a.p, hx, rr, s1, fo);
-#pragma unroll
-        for (int q = 0; q < 5; ++q) FX[q * nrx + ty * wx + tx + 1] = fo[q];
-      }
-      for (int t = tid; t < BY; t += BX * BY) {
-        const int r = t, jj = j0 + r;
-        if (jj <= n2) {
-          R ll[NQ], rr[NQ], fo[5];
-#pragma unroll
-          for (int f = 0; f < NQ; ++f) {
-            ll[f] = LX[f * BY + r];
-            rr[f] = RX[f * nrx + r * wx];
-          }
-          const R s1 = met<R>(a.geo, MFVM_MET_S1, i0 - 1, jj - 1, m - 1);
-          face_flux<0, RS, R>(a.p, ll, rr, s1, fo);
-#pragma unroll
-          for (int q = 0; q < 5; ++q) FX[q * nrx + r * wx] = fo[q];
-        }
-      }
-      __syncthreads();
-      if (own) {
-#pragma unroll
-        for (int q = 0; q < 5; ++q)
-          accm[q] = FX[q * nrx + ty * wx + tx + 1] - FX[q * nrx + ty * wx + tx];
-      }
-      __syncthreads();
-
-      // x2 phase
-      R hy[NQ];
       if (i <= n1 && j <= n2 + 1) {
         R lo[NQ];
         if (j <= n2) {
@@ -563,27 +515,42 @@ __global__ __launch_bounds__(256) void ns_stage_kernel(StageArgs a) {
             R qm[NQ], qp[NQ];
             load_q<1, R>(U, C, fs, o - a.lay.sy, visc, qm);
             load_q<1, R>(U, C, fs, o + a.lay.sy, visc, qp);
-            slope_states<1, LIM, R>(a.geo, i, j, m, visc ? 9 : 6, qm, q, qp, lo, hy);
+            slope_states<1, LIM, UNIF, R>(a.geo, i, j, m, visc ? 9 : 6, qm, q, qp, lo, hy);
           }
         } else {
-          cell_states<1, LIM, R>(a, U, C, i, j, m, lo, hy);
+          cell_states<1, LIM, UNIF, R>(a, U, C, i, j, m, lo, hy);  // x2 ghost: raw
         }
 #pragma unroll
         for (int f = 0; f < NQ; ++f) RY[f * nry + ty * BX + tx] = lo[f];
       }
-      for (int t = tid; t < 2 * BX; t += BX * BY) {
-        const int c = t % BX, side = t / BX;
-        const int ii = i0 + c;
-        if (ii <= n1) {
-          R lo[NQ], hi[NQ];
-          if (side == 0) {
-            cell_states<1, LIM, R>(a, U, C, ii, j0 - 1, m, lo, hi);
+      // helpers, phase 1: [0, 2BX) x2 tile-edge cells, [2BX, 2BX+2BY) x1 tile-edge cells
+      for (int t = tid; t < 2 * (BX + BY); t += NT) {
+        R lo[NQ], hi[NQ];
+        if (t < 2 * BX) {
+          const int c = t % BX, side = t / BX, ii = i0 + c;
+          if (ii <= n1) {
+            if (side == 0) {
+              cell_states<1, LIM, UNIF, R>(a, U, C, ii, j0 - 1, m, lo, hi);
 #pragma unroll
-            for (int f = 0; f < NQ; ++f) LY[f * BX + c] = hi[f];
-          } else if (j0 + BY <= n2 + 1) {
-            cell_states<1, LIM, R>(a, U, C, ii, j0 + BY, m, lo, hi);
+              for (int f = 0; f < NQ; ++f) LY[f * BX + c] = hi[f];
+            } else if (j0 + BY <= n2 + 1) {
+              cell_states<1, LIM, UNIF, R>(a, U, C, ii, j0 + BY, m, lo, hi);
 #pragma unroll
-            for (int f = 0; f < NQ; ++f) RY[f * nry + BY * BX + c] = lo[f];
+              for (int f = 0; f < NQ; ++f) RY[f * nry + BY * BX + c] = lo[f];
+            }
+          }
+        } else {
+          const int u = t - 2 * BX, r = u % BY, side = u / BY, jj = j0 + r;
+          if (jj <= n2) {
+            if (side == 0) {
+              cell_states<0, LIM, UNIF, R>(a, U, C, i0 - 1, jj, m, lo, hi);
+#pragma unroll
+              for (int f = 0; f < NQ; ++f) LX[f * BY + r] = hi[f];
+            } else if (i0 + BX <= n1 + 1) {
+              cell_states<0, LIM, UNIF, R>(a, U, C, i0 + BX, jj, m, lo, hi);
+#pragma unroll
+              for (int f = 0; f < NQ; ++f) RX[f * nrx + r * wx + BX] = lo[f];
+            }
           }
         }
       }
@@ -591,35 +558,54 @@ __global__ __launch_bounds__(256) void ns_stage_kernel(StageArgs a) {
       if (own) {
         R rr[NQ], fo[5];
 #pragma unroll
+        for (int f = 0; f < NQ; ++f) rr[f] = RX[f * nrx + ty * wx + tx + 1];
+        face_flux<0, RS, R>(a.p, hx, rr, g_area<UNIF, R>(a.geo, 0, i, j - 1, m - 1), fo);
+#pragma unroll
+        for (int q = 0; q < 5; ++q) FX[q * nrx + ty * wx + tx + 1] = fo[q];
+#pragma unroll
         for (int f = 0; f < NQ; ++f) rr[f] = RY[f * nry + (ty + 1) * BX + tx];
-        const R s2 = met<R>(a.geo, MFVM_MET_S2, i - 1, j, m - 1);
-        face_flux<1, RS, R>(a.p, hy, rr, s2, fo);
+        face_flux<1, RS, R>(a.p, hy, rr, g_area<UNIF, R>(a.geo, 1, i - 1, j, m - 1), fo);
 #pragma unroll
         for (int q = 0; q < 5; ++q) FY[q * nry + (ty + 1) * BX + tx] = fo[q];
       }
-      for (int t = tid; t < BX; t += BX * BY) {
-        const int c = t, ii = i0 + c;
-        if (ii <= n1) {
-          R ll[NQ], rr[NQ], fo[5];
+      // helpers, phase 2 (last threads): [0, BX) faces j0-1/2, [BX, BX+BY) faces i0-1/2
+      for (int u = NT - 1 - tid; u < BX + BY; u += NT) {
+        R ll[NQ], rr[NQ], fo[5];
+        if (u < BX) {
+          const int c = u, ii = i0 + c;
+          if (ii <= n1) {
 #pragma unroll
-          for (int f = 0; f < NQ; ++f) {
-            ll[f] = LY[f * BX + c];
-            rr[f] = RY[f * nry + c];
+            for (int f = 0; f < NQ; ++f) {
+              ll[f] = LY[f * BX + c];
+              rr[f] = RY[f * nry + c];
+            }
+            face_flux<1, RS, R>(a.p, ll, rr, g_area<UNIF, R>(a.geo, 1, ii - 1, j0 - 1, m - 1), fo);
+#pragma unroll
+            for (int q = 0; q < 5; ++q) FY[q * nry + c] = fo[q];
           }
-          const R s2 = met<R>(a.geo, MFVM_MET_S2, ii - 1, j0 - 1, m - 1);
-          face_flux<1, RS, R>(a.p, ll, rr, s2, fo);
+        } else {
+          const int r = u - BX, jj = j0 + r;
+          if (jj <= n2) {
 #pragma unroll
-          for (int q = 0; q < 5; ++q) FY[q * nry + c] = fo[q];
+            for (int f = 0; f < NQ; ++f) {
+              ll[f] = LX[f * BY + r];
+              rr[f] = RX[f * nrx + r * wx];
+            }
+            face_flux<0, RS, R>(a.p, ll, rr, g_area<UNIF, R>(a.geo, 0, i0 - 1, jj - 1, m - 1), fo);
+#pragma unroll
+            for (int q = 0; q < 5; ++q) FX[q * nrx + r * wx] = fo[q];
+          }
         }
       }
       __syncthreads();
       if (own) {
 #pragma unroll
-        for (int q = 0; q < 5; ++q)
-          accm[q] = (accm[q] + FY[q * nry + (ty + 1) * BX + tx]) - FY[q * nry + ty * BX + tx];
-        cell_source<R>(a.geo, a.p, i, j, m, cu, cu + 5, srcm, src_any);
+        for (int q = 0; q < 5; ++q) {
+          const R ax = FX[q * nrx + ty * wx + tx + 1] - FX[q * nrx + ty * wx + tx];
+          accm[q] = (ax + FY[q * nry + (ty + 1) * BX + tx]) - FY[q * nry + ty * BX + tx];
+        }
+        cell_source<UNIF, R>(a.geo, a.p, i, j, m, cu, cu + 5, srcm, src_any);
       }
-      __syncthreads();
     }
 
     // ---- finalize cell m-1 -----------------------------------------------------
@@ -628,7 +614,7 @@ __global__ __launch_bounds__(256) void ns_stage_kernel(StageArgs a) {
       R kr[5];
       R vol;
       const bool vp2 = a.geo.vpow2 != 0;
-      if (!vp2) vol = met<R>(a.geo, MFVM_MET_V, i - 1, j - 1, k - 1);
+      if (!vp2) vol = g_vol<UNIF, R>(a.geo, i, j, k);
 #pragma unroll
       for (int q = 0; q < 5; ++q) {
         const R x = -((acc_prev[q] + fsz[q]) - fsz_prev[q]);
@@ -660,7 +646,7 @@ __global__ __launch_bounds__(256) void ns_stage_kernel(StageArgs a) {
       }
       if (a.partials) {
         R rate, nu;
-        cell_cfl<R>(a.geo, a.p, i, j, k, nu_, rate, nu);
+        cell_cfl<UNIF, R>(a.geo, a.p, i, j, k, nu_, rate, nu);
         rate_max = nmax(rate_max, rate);
         nu_max = nmax(nu_max, nu);
       }
@@ -690,7 +676,7 @@ __global__ __launch_bounds__(256) void ns_stage_kernel(StageArgs a) {
 }
 
 // Standalone CFL partials (navier_stokes.py:512-583), same grid as the stage.
-template <class R>
+template <bool UNIF, class R>
 __global__ __launch_bounds__(256) void ns_cfl_kernel(Layout lay, Geo geo, NSPar p, const R* U,
                                                     double* partials, int kz) {
   extern __shared__ double smem_raw[];
@@ -705,7 +691,7 @@ __global__ __launch_bounds__(256) void ns_cfl_kernel(Layout lay, Geo geo, NSPar 
 #pragma unroll
       for (int q = 0; q < 5; ++q) u[q] = U[o + q * lay.fs];
       R rate, nu;
-      cell_cfl<R>(geo, p, i, j, k, u, rate, nu);
+      cell_cfl<UNIF, R>(geo, p, i, j, k, u, rate, nu);
       rate_max = nmax(rate_max, rate);
       nu_max = nmax(nu_max, nu);
     }
@@ -783,16 +769,23 @@ hipError_t launch_closure(const mfvm_grid* g, const mfvm_ns_params* p, const voi
   const dim3 blk(64, 4, 1);
   const dim3 grd((g->n[0] + 2 + 63) / 64, (g->n[1] + 2 + 3) / 4, g->n[2] + 2);
   const int tk = t_begin(MFVM_TK_CLOSURE, s);
-  hipLaunchKernelGGL(ns_closure_kernel<R>, grd, blk, 0, s, lay, geo, q, static_cast<const R*>(U),
-                     static_cast<R*>(C));
+  if (g->uniform)
+    hipLaunchKernelGGL((ns_closure_kernel<true, R>), grd, blk, 0, s, lay, geo, q, static_cast<const R*>(U),
+                       static_cast<R*>(C));
+  else
+    hipLaunchKernelGGL((ns_closure_kernel<false, R>), grd, blk, 0, s, lay, geo, q, static_cast<const R*>(U),
+                       static_cast<R*>(C));
   t_end(tk, s);
   return hipGetLastError();
 }
 
 template <int LIM, int RS, class R>
-static hipError_t launch_stage_t(const StageArgs& a, dim3 grd, dim3 blk, size_t lds, hipStream_t s) {
+static hipError_t launch_stage_t(const StageArgs& a, bool unif, dim3 grd, dim3 blk, size_t lds, hipStream_t s) {
   const int tk = t_begin(MFVM_TK_STAGE, s);
-  hipLaunchKernelGGL((ns_stage_kernel<LIM, RS, R>), grd, blk, lds, s, a);
+  if (unif)
+    hipLaunchKernelGGL((ns_stage_kernel<LIM, RS, true, R>), grd, blk, lds, s, a);
+  else
+    hipLaunchKernelGGL((ns_stage_kernel<LIM, RS, false, R>), grd, blk, lds, s, a);
   t_end(tk, s);
   return hipGetLastError();
 }
@@ -822,14 +815,13 @@ hipError_t launch_stage(const mfvm_grid* g, const mfvm_ns_params* p, const mfvm_
   a.kz = t.kz;
   const dim3 blk(t.bx, t.by, 1);
   const dim3 grd = ns_grid(g, t);
-  const size_t nx = (size_t)NQ * t.by * (t.bx + 1) + NQ * t.by + 5 * (size_t)t.by * (t.bx + 1);
-  const size_t ny = (size_t)NQ * (t.by + 1) * t.bx + NQ * t.bx + 5 * (size_t)(t.by + 1) * t.bx;
-  size_t lds = (nx > ny ? nx : ny) * sizeof(R);
+  const size_t nrx = (size_t)t.by * (t.bx + 1), nry = (size_t)(t.by + 1) * t.bx;
+  size_t lds = (NQ * nrx + NQ * t.by + 5 * nrx + NQ * nry + NQ * t.bx + 5 * nry) * sizeof(R);
   const size_t red = 2 * sizeof(double) * ((t.bx * t.by + 63) / 64);
   if (lds < red) lds = red;
 #define MFVM_DISPATCH_RS(LIMV)                                                     \
-  return p->riemann == MFVM_RS_HLL ? launch_stage_t<LIMV, MFVM_RS_HLL, R>(a, grd, blk, lds, s) \
-                                   : launch_stage_t<LIMV, MFVM_RS_RUSANOV, R>(a, grd, blk, lds, s);
+  return p->riemann == MFVM_RS_HLL ? launch_stage_t<LIMV, MFVM_RS_HLL, R>(a, g->uniform, grd, blk, lds, s) \
+                                   : launch_stage_t<LIMV, MFVM_RS_RUSANOV, R>(a, g->uniform, grd, blk, lds, s);
   switch (p->limiter) {
     case MFVM_LIM_NONE: MFVM_DISPATCH_RS(MFVM_LIM_NONE)
     case MFVM_LIM_MINMOD: MFVM_DISPATCH_RS(MFVM_LIM_MINMOD)
@@ -849,8 +841,12 @@ hipError_t launch_cfl(const mfvm_grid* g, const mfvm_ns_params* p, const void* U
   const dim3 blk(t.bx, t.by, 1);
   const dim3 grd = ns_grid(g, t);
   const size_t red = 2 * sizeof(double) * ((t.bx * t.by + 63) / 64);
-  hipLaunchKernelGGL(ns_cfl_kernel<R>, grd, blk, red, s, lay, geo, q, static_cast<const R*>(U),
-                     partials, t.kz);
+  if (g->uniform)
+    hipLaunchKernelGGL((ns_cfl_kernel<true, R>), grd, blk, red, s, lay, geo, q, static_cast<const R*>(U),
+                       partials, t.kz);
+  else
+    hipLaunchKernelGGL((ns_cfl_kernel<false, R>), grd, blk, red, s, lay, geo, q, static_cast<const R*>(U),
+                       partials, t.kz);
   const long long nblk = (long long)grd.x * grd.y * grd.z;
   hipLaunchKernelGGL(ns_dt_finalize_kernel, dim3(1), dim3(1024), 0, s, partials, nblk, cfl, g->smax,
                      dt_out);

@@ -28,9 +28,10 @@ E_ARG, E_HIP, E_UNSUP = -1, -2, -3
 
 class Grid(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int32 * 3), ("coord", ctypes.c_int32),
-                ("fac", ctypes.c_void_p), ("off", (ctypes.c_int32 * 3) * NMET),
-                ("lpow2", ctypes.c_int32 * 3), ("vpow2", ctypes.c_int32),
-                ("lval", ctypes.c_double * 3), ("vval", ctypes.c_double),
+                ("fac", ctypes.c_void_p), ("fstride", ctypes.c_int32),
+                ("uniform", ctypes.c_int32), ("lpow2", ctypes.c_int32 * 3),
+                ("vpow2", ctypes.c_int32), ("lval", ctypes.c_double * 3),
+                ("vval", ctypes.c_double), ("sval", ctypes.c_double * 3),
                 ("smax", ctypes.c_double), ("lmin", ctypes.c_double * 3)]
 
 

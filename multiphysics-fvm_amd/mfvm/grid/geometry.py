@@ -130,28 +130,33 @@ class GridDesc:
         self.coordinate_system = coordinate_system
         self.n = n
         self.device = torch.device(device)
-        chunks, off = [], []
-        pos = 0
+        stride = max(n) + 1
+        table = np.zeros((_lib.NMET * 3, stride), dtype=np.float64)
         for m in range(_lib.NMET):
-            o = []
-            for arr in F[m]:
-                o.append(pos)
-                chunks.append(np.asarray(arr, dtype=np.float64))
-                pos += len(arr)
-            off.append(o)
+            for a, arr in enumerate(F[m]):
+                table[3 * m + a, :len(arr)] = arr
         self.factors_host = F
-        self.fac = torch.from_numpy(np.concatenate(chunks)).to(self.device)
+        self.fac = torch.from_numpy(table.reshape(-1)).to(self.device)
         d = _lib.Grid()
         d.n[:] = list(n)
         d.coord = {"cartesian": _lib.CART, "cylindrical": _lib.CYL, "spherical": _lib.SPH}[coordinate_system]
         d.fac = self.fac.data_ptr()
-        for m in range(_lib.NMET):
-            d.off[m][:] = off[m]
+        d.fstride = stride
         for a in range(3):
             flag, val = _uniform_pow2(*F[_lib.MET_L1 + a])
             d.lpow2[a] = flag
             d.lval[a] = val
         d.vpow2, d.vval = _uniform_pow2(*F[_lib.MET_V])
+        uni = coordinate_system == "cartesian" and all(
+            np.all(x == x[0]) for m in range(_lib.MET_V, _lib.MET_L3 + 1) for x in F[m])
+        d.uniform = int(uni)
+        if uni:
+            def val(m):
+                a, b, c = F[m]
+                return (float(a[0]) * float(b[0])) * float(c[0])
+            d.lval[:] = [val(_lib.MET_L1), val(_lib.MET_L2), val(_lib.MET_L3)]
+            d.vval = val(_lib.MET_V)
+            d.sval[:] = [val(_lib.MET_S1), val(_lib.MET_S2), val(_lib.MET_S3)]
         d.smax = self._smax(F)
         for a in range(3):
             fa, fb, fc = F[_lib.MET_L1 + a]

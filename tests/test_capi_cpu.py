@@ -43,6 +43,7 @@ def test_abi_version_and_error_paths():
     g.n[:] = [4, 4, 4]
     g.coord = 7
     g.fac = 1
+    g.fstride = 5
     rc = _lib.lib.mfvm_ns_closure(ctypes.byref(g), ctypes.byref(p), None, None, None)
     assert rc == _lib.E_ARG and "coordinate system" in _lib.lib.mfvm_last_error().decode()
     g.coord = 0
