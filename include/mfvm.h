@@ -122,6 +122,22 @@ typedef struct mfvm_ns_params {  /* navier_stokes.py: params + space_schemes */
 #define MFVM_ACC_SET 1    /* kacc = k */
 #define MFVM_ACC_ADD 2    /* kacc = kacc + acc_coef*k */
 
+/* Compensated FP32 stages (dtype f32 with a compensation word per value):
+ * U0 = hi + lo, stage states U0 + d with an FP32 increment d.
+ *   SET:        d = (c0*dt)*k               SSP: d = c0*(d + dt*k)
+ *   FINAL_SET:  U0 += (c0*dt)*k             FINAL_SSP: U0 += c0*(d + dt*k)
+ *   FINAL_ACC6: U0 += (dt/6)*(kacc + k)
+ * Non-final stages write the next stage state hi + (lo + d) to snext; final
+ * stages add the increment to hi + lo in FP64 and store the rounded sum and
+ * its FP32 remainder in (out, lo).  No reference counterpart (the reference is
+ * FP64 only): this is the reduced-precision mode of the build. */
+#define MFVM_COMP_NONE 0
+#define MFVM_COMP_SET 1
+#define MFVM_COMP_SSP 2
+#define MFVM_COMP_FINAL_SET 3
+#define MFVM_COMP_FINAL_SSP 4
+#define MFVM_COMP_FINAL_ACC6 5
+
 typedef struct mfvm_stage {
   const void* U;                /* stage input (ghosts filled): stencil reads */
   const void* C;                /* closure of U (mfvm_ns_closure) */
@@ -134,6 +150,10 @@ typedef struct mfvm_stage {
   double dt;                    /* used when dt_dev is NULL */
   const double* dt_dev;         /* device scalar dt, or NULL */
   double* partials;             /* non-NULL: reduce CFL rates of out */
+  void* lo;                     /* compensated stages: compensation of U0 (ghosted) */
+  void* dinc;                   /* compensated stages: increment (interior) */
+  void* snext;                  /* compensated stages: next stage state (ghosted) */
+  int32_t comp;                 /* MFVM_COMP_* (requires dtype f32) */
 } mfvm_stage;
 
 typedef struct mfvm_heat_params {  /* heat.py: params + space_schemes */
@@ -237,6 +257,24 @@ int mfvm_ns_advance(const mfvm_grid* g, const mfvm_ns_params* p, const mfvm_ns_w
                     int32_t integrator, double cfl, int32_t nsteps, void* stream);
 
 int64_t mfvm_ns_partials_size(const mfvm_grid* g, const mfvm_ns_params* p);
+
+/* Compensated FP32 Navier-Stokes steps (FP32 reconstruction and fluxes,
+ * FP64 final-stage accumulation and CFL).  No reference counterpart: the
+ * reduced-precision form beside the FP64 drop-in. */
+typedef struct mfvm_ns_comp_work {
+  void* hi;                     /* ghosted f32 state (rounded values), advanced in place */
+  void* lo;                     /* ghosted f32 compensation words */
+  void* Sa;                     /* ghosted f32 stage states */
+  void* Sb;
+  void* C;                      /* ghosted f32 closure, 7 fields; ghost entries must be 0 */
+  void* dinc;                   /* interior f32 increment */
+  void* kacc;                   /* interior f32 accumulator */
+  double* partials;             /* >= mfvm_ns_partials_size() doubles */
+  double* dts;                  /* device, nsteps+1 entries */
+} mfvm_ns_comp_work;
+
+int mfvm_ns_comp_advance(const mfvm_grid* g, const mfvm_ns_params* p, const mfvm_ns_comp_work* w,
+                         int32_t integrator, double cfl, int32_t nsteps, void* stream);
 
 /* ---- heat equation (src/physics/heat.py) -------------------------------------- */
 

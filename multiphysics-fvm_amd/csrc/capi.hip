@@ -9,7 +9,7 @@ namespace mfvm {
 // Defined in ns.hip / bc.hip / heat.hip / maxwell.hip.
 template <class R> hipError_t launch_closure(const mfvm_grid*, const mfvm_ns_params*, const void*, void*, hipStream_t);
 template <class R> hipError_t launch_stage(const mfvm_grid*, const mfvm_ns_params*, const mfvm_stage*, hipStream_t);
-template <class R> hipError_t launch_cfl(const mfvm_grid*, const mfvm_ns_params*, const void*, double, double*, double*, hipStream_t);
+template <class R> hipError_t launch_cfl(const mfvm_grid*, const mfvm_ns_params*, const void*, const void*, double, double*, double*, hipStream_t);
 hipError_t launch_dt_finalize(const mfvm_grid*, const mfvm_ns_params*, const double*, double, double*, hipStream_t);
 long long ns_nblocks(const mfvm_grid*, const mfvm_ns_params*);
 template <class R> hipError_t launch_bcs(const mfvm_grid*, int, const int32_t*, const void*, void*, hipStream_t);
@@ -205,6 +205,20 @@ int mfvm_ns_stage(const mfvm_grid* g, const mfvm_ns_params* p, const mfvm_stage*
   int rc = check_ns(g, p);
   if (rc) return rc;
   if (!st || !st->U || !st->C) return fail(MFVM_E_ARG, "ns_stage: NULL input");
+  if (st->comp != MFVM_COMP_NONE) {
+    if (st->comp < MFVM_COMP_SET || st->comp > MFVM_COMP_FINAL_ACC6) return fail(MFVM_E_ARG, "ns_stage: bad comp mode");
+    if (p->dtype != MFVM_F32) return fail(MFVM_E_ARG, "ns_stage: compensated stages need dtype f32");
+    const bool fin = st->comp >= MFVM_COMP_FINAL_SET;
+    if (!st->U0 || !st->lo || !st->dinc) return fail(MFVM_E_ARG, "ns_stage: compensated stage needs U0, lo, dinc");
+    if (fin && !st->out) return fail(MFVM_E_ARG, "ns_stage: compensated final stage needs out");
+    if (!fin && !st->snext) return fail(MFVM_E_ARG, "ns_stage: compensated stage needs snext");
+    if (fin && st->out == st->U) return fail(MFVM_E_ARG, "ns_stage: output must not alias the stencil input");
+    if (!fin && st->snext == st->U) return fail(MFVM_E_ARG, "ns_stage: snext must not alias the stencil input");
+    if ((st->acc != MFVM_ACC_NONE || st->comp == MFVM_COMP_FINAL_ACC6) && !st->kacc)
+      return fail(MFVM_E_ARG, "ns_stage: NULL accumulator");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    return hip_check(launch_stage<float>(g, p, st, s), "ns_stage");
+  }
   if (st->update < MFVM_UPD_NONE || st->update > MFVM_UPD_ACC6) return fail(MFVM_E_ARG, "ns_stage: bad update mode");
   if (st->acc < MFVM_ACC_NONE || st->acc > MFVM_ACC_ADD) return fail(MFVM_E_ARG, "ns_stage: bad accumulator mode");
   if (st->update != MFVM_UPD_NONE && !st->out) return fail(MFVM_E_ARG, "ns_stage: NULL output");
@@ -242,8 +256,8 @@ int mfvm_ns_dt(const mfvm_grid* g, const mfvm_ns_params* p, const void* U, doubl
   if (rc) return rc;
   if (!U || !partials || !dt_out) return fail(MFVM_E_ARG, "ns_dt: NULL buffer");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  return hip_check(p->dtype == MFVM_F64 ? launch_cfl<double>(g, p, U, cfl, partials, dt_out, s)
-                                        : launch_cfl<float>(g, p, U, cfl, partials, dt_out, s),
+  return hip_check(p->dtype == MFVM_F64 ? launch_cfl<double>(g, p, U, nullptr, cfl, partials, dt_out, s)
+                                        : launch_cfl<float>(g, p, U, nullptr, cfl, partials, dt_out, s),
                    "ns_dt");
 }
 
@@ -389,6 +403,86 @@ int mfvm_ns_advance(const mfvm_grid* g, const mfvm_ns_params* p, const mfvm_ns_w
                         "ns_advance")))
       return rc;
   }
+  return MFVM_OK;
+}
+
+// ---- compensated FP32 Navier-Stokes ----
+int mfvm_ns_comp_advance(const mfvm_grid* g, const mfvm_ns_params* p, const mfvm_ns_comp_work* w,
+                         int32_t integrator, double cfl, int32_t nsteps, void* stream) {
+  int rc = check_ns(g, p);
+  if (rc) return rc;
+  if (p->dtype != MFVM_F32) return fail(MFVM_E_ARG, "ns_comp_advance: dtype must be f32");
+  if (!w || !w->hi || !w->lo || !w->Sa || !w->Sb || !w->C || !w->dinc || !w->kacc || !w->partials || !w->dts)
+    return fail(MFVM_E_ARG, "ns_comp_advance: NULL work buffer");
+  if (integrator < MFVM_INT_EULER || integrator > MFVM_INT_RK4) return fail(MFVM_E_ARG, "unknown integrator");
+  if (nsteps < 0) return fail(MFVM_E_ARG, "nsteps must be >= 0");
+  if (nsteps == 0) return MFVM_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int32_t* bc = &p->bc[0][0];
+  if ((rc = hip_check(launch_cfl<float>(g, p, w->hi, w->lo, cfl, w->partials, w->dts, s), "ns_comp_advance")))
+    return rc;
+  struct S {
+    void* in;
+    void* next;
+    int comp, acc;
+    double c0, ac;
+  };
+  S seq[4];
+  int ns = 0;
+  switch (integrator) {
+    case MFVM_INT_EULER:
+      seq[0] = {w->hi, nullptr, MFVM_COMP_FINAL_SET, MFVM_ACC_NONE, 1.0, 0};
+      ns = 1;
+      break;
+    case MFVM_INT_RK3:
+      seq[0] = {w->hi, w->Sa, MFVM_COMP_SET, MFVM_ACC_SET, 0.5, 0};
+      seq[1] = {w->Sa, w->Sb, MFVM_COMP_SET, MFVM_ACC_ADD, 1.0, 4.0};
+      seq[2] = {w->Sb, nullptr, MFVM_COMP_FINAL_ACC6, MFVM_ACC_NONE, 0, 0};
+      ns = 3;
+      break;
+    case MFVM_INT_SSPRK3:
+      seq[0] = {w->hi, w->Sa, MFVM_COMP_SET, MFVM_ACC_NONE, 1.0, 0};
+      seq[1] = {w->Sa, w->Sb, MFVM_COMP_SSP, MFVM_ACC_NONE, 0.25, 0};
+      seq[2] = {w->Sb, nullptr, MFVM_COMP_FINAL_SSP, MFVM_ACC_NONE, 2.0 / 3.0, 0};
+      ns = 3;
+      break;
+    default:
+      seq[0] = {w->hi, w->Sa, MFVM_COMP_SET, MFVM_ACC_SET, 0.5, 0};
+      seq[1] = {w->Sa, w->Sb, MFVM_COMP_SET, MFVM_ACC_ADD, 0.5, 2.0};
+      seq[2] = {w->Sb, w->Sa, MFVM_COMP_SET, MFVM_ACC_ADD, 1.0, 2.0};
+      seq[3] = {w->Sa, nullptr, MFVM_COMP_FINAL_ACC6, MFVM_ACC_NONE, 0, 0};
+      ns = 4;
+      break;
+  }
+  for (int n = 0; n < nsteps; ++n) {
+    for (int q = 0; q < ns; ++q) {
+      if ((rc = mfvm_apply_bcs(g, 5, bc, MFVM_F32, seq[q].in, seq[q].in, stream))) return rc;
+      if ((rc = mfvm_ns_closure(g, p, seq[q].in, w->C, stream))) return rc;
+      const bool fin = q == ns - 1;
+      mfvm_stage st = {};
+      st.U = seq[q].in;
+      st.C = w->C;
+      st.U0 = w->hi;
+      st.out = fin ? (integrator == MFVM_INT_EULER ? w->Sa : w->hi) : nullptr;
+      st.kacc = w->kacc;
+      st.update = MFVM_UPD_NONE;
+      st.acc = seq[q].acc;
+      st.c0 = seq[q].c0;
+      st.acc_coef = seq[q].ac;
+      st.dt_dev = w->dts + n;
+      st.partials = fin ? w->partials : nullptr;
+      st.lo = w->lo;
+      st.dinc = w->dinc;
+      st.snext = seq[q].next;
+      st.comp = seq[q].comp;
+      if ((rc = mfvm_ns_stage(g, p, &st, stream))) return rc;
+    }
+    if (integrator == MFVM_INT_EULER && (rc = mfvm_copy_interior(g, 5, MFVM_F32, w->Sa, w->hi, stream)))
+      return rc;
+    if ((rc = hip_check(launch_dt_finalize(g, p, w->partials, cfl, w->dts + n + 1, s), "ns_comp_advance")))
+      return rc;
+  }
+  if (ns > 1 && (rc = mfvm_apply_bcs(g, 5, bc, MFVM_F32, seq[ns - 1].in, w->hi, stream))) return rc;
   return MFVM_OK;
 }
 

@@ -52,6 +52,12 @@ struct StageArgs {
   const double* dt_dev;
   double* partials;
   int kz;
+  // compensated FP32 stages (MFVM_COMP_*): lo = compensation of U0 (ghosted),
+  // dinc = increment over U0 (interior), snext = next stage state (ghosted)
+  void* lo;
+  void* dinc;
+  void* snext;
+  int comp;
 };
 
 // ---------------------------------------------------------------------------
@@ -369,8 +375,8 @@ MFVM_DEV void block_max2(R rate, R nu, double* partials, double* sred) {
 // ---------------------------------------------------------------------------
 // Fused stage kernel
 // ---------------------------------------------------------------------------
-template <int LIM, int RS, bool UNIF, class R>
-__global__ __launch_bounds__(256) void ns_stage_kernel(StageArgs a) {
+template <int LIM, int RS, bool UNIF, class R, bool COMP>
+MFVM_DEV void ns_stage_body(const StageArgs& a) {
   extern __shared__ double smem_raw[];
   R* sm = reinterpret_cast<R*>(smem_raw);
   const R* __restrict__ U = static_cast<const R*>(a.U);
@@ -408,7 +414,7 @@ __global__ __launch_bounds__(256) void ns_stage_kernel(StageArgs a) {
   R fsz_prev[5];            // F.S of face m-3/2
   R acc_prev[5], src_prev[5];
   bool src_any_prev = false;
-  R rate_max = R(0), nu_max = R(0);
+  double rate_max = 0.0, nu_max = 0.0;
 
   auto load12 = [&](int k, R* w) {
     const long long o = a.lay.at(i, j, k);
@@ -626,29 +632,71 @@ __global__ __launch_bounds__(256) void ns_stage_kernel(StageArgs a) {
       R* out = static_cast<R*>(a.out);
       R* ka = static_cast<R*>(a.kacc);
       const R* U0 = static_cast<const R*>(a.U0);
-      R nu_[5];
-      const R dtr = R(dt);
+      if constexpr (!COMP) {
+        R nu_[5];
+        const R dtr = R(dt);
 #pragma unroll
-      for (int q = 0; q < 5; ++q) {
-        const R uq = zp[q];  // stage input U at cell m-1
-        R o = uq;
-        switch (a.update) {
-          case MFVM_UPD_EULER: o = uq + dtr * kr[q]; break;
-          case MFVM_UPD_BASE: o = U0[oc + q * fs] + R(a.c0 * dt) * kr[q]; break;
-          case MFVM_UPD_SSP: o = (R(a.c0) * U0[oc + q * fs]) + (R(a.c1) * (uq + dtr * kr[q])); break;
-          case MFVM_UPD_ACC6: o = U0[oc + q * fs] + R(dt / 6.0) * (ka[oi + q * a.il.fs] + kr[q]); break;
-          default: break;
+        for (int q = 0; q < 5; ++q) {
+          const R uq = zp[q];  // stage input U at cell m-1
+          R o = uq;
+          switch (a.update) {
+            case MFVM_UPD_EULER: o = uq + dtr * kr[q]; break;
+            case MFVM_UPD_BASE: o = U0[oc + q * fs] + R(a.c0 * dt) * kr[q]; break;
+            case MFVM_UPD_SSP: o = (R(a.c0) * U0[oc + q * fs]) + (R(a.c1) * (uq + dtr * kr[q])); break;
+            case MFVM_UPD_ACC6: o = U0[oc + q * fs] + R(dt / 6.0) * (ka[oi + q * a.il.fs] + kr[q]); break;
+            default: break;
+          }
+          nu_[q] = o;
+          if (a.update != MFVM_UPD_NONE) out[oc + q * fs] = o;
+          if (a.acc == MFVM_ACC_SET) ka[oi + q * a.il.fs] = kr[q];
+          else if (a.acc == MFVM_ACC_ADD) ka[oi + q * a.il.fs] = ka[oi + q * a.il.fs] + R(a.acc_coef) * kr[q];
         }
-        nu_[q] = o;
-        if (a.update != MFVM_UPD_NONE) out[oc + q * fs] = o;
-        if (a.acc == MFVM_ACC_SET) ka[oi + q * a.il.fs] = kr[q];
-        else if (a.acc == MFVM_ACC_ADD) ka[oi + q * a.il.fs] = ka[oi + q * a.il.fs] + R(a.acc_coef) * kr[q];
-      }
-      if (a.partials) {
-        R rate, nu;
-        cell_cfl<UNIF, R>(a.geo, a.p, i, j, k, nu_, rate, nu);
-        rate_max = nmax(rate_max, rate);
-        nu_max = nmax(nu_max, nu);
+        if (a.partials) {
+          R rate, nu;
+          cell_cfl<UNIF, R>(a.geo, a.p, i, j, k, nu_, rate, nu);
+          rate_max = nmax(rate_max, double(rate));
+          nu_max = nmax(nu_max, double(nu));
+        }
+      } else {
+        // Compensated update: U0 = hi + lo (two FP32 words), stage states are
+        // U0 + d with an FP32 increment d; the last stage adds its increment
+        // to hi + lo in FP64 and splits the sum back into (hi, lo).
+        R* lo = static_cast<R*>(a.lo);
+        R* dv = static_cast<R*>(a.dinc);
+        R* sn = static_cast<R*>(a.snext);
+        const bool fin = a.comp >= MFVM_COMP_FINAL_SET;
+        double nw[5];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+          const R kq = kr[q];
+          const long long og = oc + q * fs, ii = oi + q * a.il.fs;
+          const R h = U0[og], l = lo[og];
+          if (!fin) {
+            R d;
+            if (a.comp == MFVM_COMP_SET) d = R(a.c0 * dt) * kq;
+            else d = R(a.c0) * (dv[ii] + R(dt) * kq);
+            dv[ii] = d;
+            sn[og] = h + (l + d);
+          } else {
+            double inc;
+            if (a.comp == MFVM_COMP_FINAL_SET) inc = (a.c0 * dt) * double(kq);
+            else if (a.comp == MFVM_COMP_FINAL_SSP) inc = a.c0 * (double(dv[ii]) + dt * double(kq));
+            else inc = (dt / 6.0) * (double(ka[ii]) + double(kq));
+            const double v = (double(h) + double(l)) + inc;
+            const R nh = R(v);
+            out[og] = nh;
+            lo[og] = R(v - double(nh));
+            nw[q] = v;
+          }
+          if (a.acc == MFVM_ACC_SET) ka[ii] = kq;
+          else if (a.acc == MFVM_ACC_ADD) ka[ii] = ka[ii] + R(a.acc_coef) * kq;
+        }
+        if (fin && a.partials) {
+          double rate, nu;
+          cell_cfl<UNIF, double>(a.geo, a.p, i, j, k, nw, rate, nu);
+          rate_max = nmax(rate_max, rate);
+          nu_max = nmax(nu_max, nu);
+        }
       }
     }
 
@@ -672,31 +720,52 @@ __global__ __launch_bounds__(256) void ns_stage_kernel(StageArgs a) {
     }
   }
 
-  if (a.partials) block_max2<R>(rate_max, nu_max, a.partials, reinterpret_cast<double*>(smem_raw));
+  if (a.partials) block_max2<double>(rate_max, nu_max, a.partials, reinterpret_cast<double*>(smem_raw));
+}
+
+template <int LIM, int RS, bool UNIF, class R>
+__global__ __launch_bounds__(256) void ns_stage_kernel(StageArgs a) {
+  ns_stage_body<LIM, RS, UNIF, R, false>(a);
+}
+
+// Compensated FP32 stage: half the registers of the FP64 kernel, two waves per SIMD.
+template <int LIM, int RS, bool UNIF>
+__global__ __launch_bounds__(256, 2) void ns_stage_comp_kernel(StageArgs a) {
+  ns_stage_body<LIM, RS, UNIF, float, true>(a);
 }
 
 // Standalone CFL partials (navier_stokes.py:512-583), same grid as the stage.
 template <bool UNIF, class R>
-__global__ __launch_bounds__(256) void ns_cfl_kernel(Layout lay, Geo geo, NSPar p, const R* U,
+__global__ __launch_bounds__(256) void ns_cfl_kernel(Layout lay, Geo geo, NSPar p, const R* U, const R* lo,
                                                     double* partials, int kz) {
   extern __shared__ double smem_raw[];
   const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
   const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
   const int kb = 1 + blockIdx.z * kz, ke = min(kb + kz, lay.n3 + 1);
-  R rate_max = R(0), nu_max = R(0);
+  double rate_max = 0.0, nu_max = 0.0;
   if (i <= lay.n1 && j <= lay.n2) {
     for (int k = kb; k < ke; ++k) {
       const long long o = lay.at(i, j, k);
-      R u[5];
+      if (lo) {  // compensated FP32 state: rates in FP64 of hi + lo
+        double u[5];
 #pragma unroll
-      for (int q = 0; q < 5; ++q) u[q] = U[o + q * lay.fs];
-      R rate, nu;
-      cell_cfl<UNIF, R>(geo, p, i, j, k, u, rate, nu);
-      rate_max = nmax(rate_max, rate);
-      nu_max = nmax(nu_max, nu);
+        for (int q = 0; q < 5; ++q) u[q] = double(U[o + q * lay.fs]) + double(lo[o + q * lay.fs]);
+        double rate, nu;
+        cell_cfl<UNIF, double>(geo, p, i, j, k, u, rate, nu);
+        rate_max = nmax(rate_max, rate);
+        nu_max = nmax(nu_max, nu);
+      } else {
+        R u[5];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) u[q] = U[o + q * lay.fs];
+        R rate, nu;
+        cell_cfl<UNIF, R>(geo, p, i, j, k, u, rate, nu);
+        rate_max = nmax(rate_max, double(rate));
+        nu_max = nmax(nu_max, double(nu));
+      }
     }
   }
-  block_max2<R>(rate_max, nu_max, partials, smem_raw);
+  block_max2<double>(rate_max, nu_max, partials, smem_raw);
 }
 
 // dt = cfl / (max rate_c + (2*max nu) * smax)   (navier_stokes.py:571-583)
@@ -782,7 +851,12 @@ hipError_t launch_closure(const mfvm_grid* g, const mfvm_ns_params* p, const voi
 template <int LIM, int RS, class R>
 static hipError_t launch_stage_t(const StageArgs& a, bool unif, dim3 grd, dim3 blk, size_t lds, hipStream_t s) {
   const int tk = t_begin(MFVM_TK_STAGE, s);
-  if (unif)
+  if (a.comp != MFVM_COMP_NONE) {
+    if (unif)
+      hipLaunchKernelGGL((ns_stage_comp_kernel<LIM, RS, true>), grd, blk, lds, s, a);
+    else
+      hipLaunchKernelGGL((ns_stage_comp_kernel<LIM, RS, false>), grd, blk, lds, s, a);
+  } else if (unif)
     hipLaunchKernelGGL((ns_stage_kernel<LIM, RS, true, R>), grd, blk, lds, s, a);
   else
     hipLaunchKernelGGL((ns_stage_kernel<LIM, RS, false, R>), grd, blk, lds, s, a);
@@ -813,6 +887,10 @@ hipError_t launch_stage(const mfvm_grid* g, const mfvm_ns_params* p, const mfvm_
   a.dt_dev = st->dt_dev;
   a.partials = st->partials;
   a.kz = t.kz;
+  a.lo = st->lo;
+  a.dinc = st->dinc;
+  a.snext = st->snext;
+  a.comp = st->comp;
   const dim3 blk(t.bx, t.by, 1);
   const dim3 grd = ns_grid(g, t);
   const size_t nrx = (size_t)t.by * (t.bx + 1), nry = (size_t)(t.by + 1) * t.bx;
@@ -832,7 +910,7 @@ hipError_t launch_stage(const mfvm_grid* g, const mfvm_ns_params* p, const mfvm_
 }
 
 template <class R>
-hipError_t launch_cfl(const mfvm_grid* g, const mfvm_ns_params* p, const void* U, double cfl,
+hipError_t launch_cfl(const mfvm_grid* g, const mfvm_ns_params* p, const void* U, const void* lo, double cfl,
                       double* partials, double* dt_out, hipStream_t s) {
   const Tile t = ns_tile(g, p);
   const Layout lay = make_layout(g->n[0], g->n[1], g->n[2]);
@@ -843,10 +921,10 @@ hipError_t launch_cfl(const mfvm_grid* g, const mfvm_ns_params* p, const void* U
   const size_t red = 2 * sizeof(double) * ((t.bx * t.by + 63) / 64);
   if (g->uniform)
     hipLaunchKernelGGL((ns_cfl_kernel<true, R>), grd, blk, red, s, lay, geo, q, static_cast<const R*>(U),
-                       partials, t.kz);
+                       static_cast<const R*>(lo), partials, t.kz);
   else
     hipLaunchKernelGGL((ns_cfl_kernel<false, R>), grd, blk, red, s, lay, geo, q, static_cast<const R*>(U),
-                       partials, t.kz);
+                       static_cast<const R*>(lo), partials, t.kz);
   const long long nblk = (long long)grd.x * grd.y * grd.z;
   hipLaunchKernelGGL(ns_dt_finalize_kernel, dim3(1), dim3(1024), 0, s, partials, nblk, cfl, g->smax,
                      dt_out);
@@ -865,7 +943,7 @@ template hipError_t launch_closure<double>(const mfvm_grid*, const mfvm_ns_param
 template hipError_t launch_closure<float>(const mfvm_grid*, const mfvm_ns_params*, const void*, void*, hipStream_t);
 template hipError_t launch_stage<double>(const mfvm_grid*, const mfvm_ns_params*, const mfvm_stage*, hipStream_t);
 template hipError_t launch_stage<float>(const mfvm_grid*, const mfvm_ns_params*, const mfvm_stage*, hipStream_t);
-template hipError_t launch_cfl<double>(const mfvm_grid*, const mfvm_ns_params*, const void*, double, double*, double*, hipStream_t);
-template hipError_t launch_cfl<float>(const mfvm_grid*, const mfvm_ns_params*, const void*, double, double*, double*, hipStream_t);
+template hipError_t launch_cfl<double>(const mfvm_grid*, const mfvm_ns_params*, const void*, const void*, double, double*, double*, hipStream_t);
+template hipError_t launch_cfl<float>(const mfvm_grid*, const mfvm_ns_params*, const void*, const void*, double, double*, double*, hipStream_t);
 
 }  // namespace mfvm

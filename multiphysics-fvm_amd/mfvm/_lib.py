@@ -8,7 +8,7 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "lib", "libmfvm.so")
+LIB_PATH = os.environ.get("MFVM_LIB") or os.path.join(_HERE, "lib", "libmfvm.so")
 
 NMET = 19
 (MET_V, MET_S1, MET_S2, MET_S3, MET_L1, MET_L2, MET_L3, MET_H1, MET_H2, MET_H3,
@@ -48,7 +48,15 @@ class Stage(ctypes.Structure):
                 ("update", ctypes.c_int32), ("acc", ctypes.c_int32),
                 ("c0", ctypes.c_double), ("c1", ctypes.c_double),
                 ("acc_coef", ctypes.c_double), ("dt", ctypes.c_double),
-                ("dt_dev", ctypes.c_void_p), ("partials", ctypes.c_void_p)]
+                ("dt_dev", ctypes.c_void_p), ("partials", ctypes.c_void_p),
+                ("lo", ctypes.c_void_p), ("dinc", ctypes.c_void_p), ("snext", ctypes.c_void_p),
+                ("comp", ctypes.c_int32)]
+
+
+class NSCompWork(ctypes.Structure):
+    _fields_ = [("hi", ctypes.c_void_p), ("lo", ctypes.c_void_p), ("Sa", ctypes.c_void_p),
+                ("Sb", ctypes.c_void_p), ("C", ctypes.c_void_p), ("dinc", ctypes.c_void_p),
+                ("kacc", ctypes.c_void_p), ("partials", ctypes.c_void_p), ("dts", ctypes.c_void_p)]
 
 
 class HeatParams(ctypes.Structure):
@@ -89,6 +97,8 @@ SIGNATURES = [
     ("mfvm_ns_advance", ctypes.c_int, [_G, _NP, ctypes.POINTER(NSWork), ctypes.c_int32,
                                        ctypes.c_double, ctypes.c_int32, _P]),
     ("mfvm_ns_partials_size", ctypes.c_int64, [_G, _NP]),
+    ("mfvm_ns_comp_advance", ctypes.c_int, [_G, _NP, ctypes.POINTER(NSCompWork), ctypes.c_int32,
+                                            ctypes.c_double, ctypes.c_int32, _P]),
     ("mfvm_heat_rhs", ctypes.c_int, [_G, ctypes.POINTER(HeatParams), _P, _P, _P]),
     ("mfvm_heat_dt", ctypes.c_int, [_G, ctypes.POINTER(HeatParams), ctypes.c_double, _P, _P]),
     ("mfvm_heat_step", ctypes.c_int, [_G, ctypes.POINTER(HeatParams), ctypes.POINTER(NSWork),

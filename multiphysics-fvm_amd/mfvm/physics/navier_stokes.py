@@ -119,3 +119,54 @@ def advance(U, grid, params, space_schemes, bcs, integrator, cfl, nsteps, tile=N
     _lib.check(_lib.lib.mfvm_ns_advance(g.ptr(), ctypes.byref(p), ctypes.byref(w), integrator,
                                         float(cfl), int(nsteps), c.stream()))
     return dts
+
+
+# ----------------------------------------------------------------------------
+# Compensated FP32 mode (no reference counterpart: the reduced-precision form)
+# ----------------------------------------------------------------------------
+
+def to_compensated(U64):
+    """FP32 state (hi) with its compensation words (lo) from an FP64 state:
+    hi = fl32(u), lo = fl32(u - hi), so hi + lo carries ~48 significant bits."""
+    c.require_state(U64, VARS)
+    if U64.dtype != torch.float64:
+        raise TypeError("to_compensated expects a float64 state")
+    cfg = {"nx1": U64.n[0], "nx2": U64.n[1], "nx3": U64.n[2]}
+    U = State(VARS, cfg, dtype=torch.float32, device=U64.device)
+    U.buf.copy_(U64.buf.to(torch.float32))
+    lo = U.scratch("lo", tuple(U.buf.shape))
+    lo.copy_((U64.buf - U.buf.to(torch.float64)).to(torch.float32))
+    return U
+
+
+def compensated_value(U):
+    """hi + lo in FP64 (the state the compensated path represents)."""
+    lo = U.scratch("lo", tuple(U.buf.shape))
+    return U.buf.to(torch.float64) + lo.to(torch.float64)
+
+
+def advance_compensated(U, grid, params, space_schemes, bcs, integrator, cfl, nsteps, tile=None):
+    """``nsteps`` steps with FP32 reconstruction and fluxes and the FP64
+    compensated accumulation (mfvm_ns_comp_advance).  U from to_compensated."""
+    c.require_state(U, VARS)
+    if U.dtype != torch.float32:
+        raise TypeError("advance_compensated expects the float32 state of to_compensated")
+    g = c.require_grid(grid, U)
+    p = make_params(U, params, space_schemes, bcs, tile)
+    n1, n2, n3 = U.n
+    w = _lib.NSCompWork()
+    w.hi = U.buf.data_ptr()
+    w.lo = U.scratch("lo", tuple(U.buf.shape)).data_ptr()
+    w.Sa = U.scratch("S1", tuple(U.buf.shape)).data_ptr()
+    w.Sb = U.scratch("S2", tuple(U.buf.shape)).data_ptr()
+    w.C = _closure_buf(U).data_ptr()
+    w.dinc = U.scratch("dinc", (5, n3, n2, n1)).data_ptr()
+    w.kacc = U.scratch("kacc", (5, n3, n2, n1)).data_ptr()
+    n = _lib.lib.mfvm_ns_partials_size(g.ptr(), ctypes.byref(p))
+    partials = U.scratch64("partials", max(int(n), 1))
+    dts = torch.zeros(nsteps + 1, dtype=torch.float64, device=U.device)
+    w.partials = partials.data_ptr()
+    w.dts = dts.data_ptr()
+    _lib.check(_lib.lib.mfvm_ns_comp_advance(g.ptr(), ctypes.byref(p), ctypes.byref(w), integrator,
+                                             float(cfl), int(nsteps), c.stream()))
+    return dts

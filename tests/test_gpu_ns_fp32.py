@@ -1,0 +1,82 @@
+"""FP32-compensated Navier-Stokes path vs the FP64 path / oracle.
+
+Tolerance: L1 (mean absolute) error <= 1e-6 per field against the float64
+result, on the 2D Kelvin-Helmholtz configuration (200 SSPRK3 steps) and a
+3D Rayleigh-Taylor case.  L-inf is printed for every field.  Plain FP32
+(no compensation) is run beside it for comparison and only reported.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(n, kind):
+    from mfvm.grid import boundary_conditions as bc
+    from mfvm.grid.geometry import build_grid
+    from mfvm.numerics import diffusive_solvers, reconstructors, riemann_solvers, slope_limiters
+    from mfvm.physics import navier_stokes as ns
+    cfg = {"x1_min": 0.0, "x1_max": 1.0, "nx1": n[0], "x2_min": 0.0, "x2_max": 1.0, "nx2": n[1],
+           "x3_min": 0.0, "x3_max": 1.0, "nx3": n[2]}
+    grid = build_grid("cartesian", cfg)
+    U = ns.allocate_u(cfg)
+    rng = np.random.default_rng(0)
+    if kind == "kh":
+        x = (np.arange(n[0]) + 0.5) / n[0]
+        y = (np.arange(n[1]) + 0.5) / n[1]
+        X, Y = np.meshgrid(x, y, indexing="ij")
+        X, Y = X[:, :, None], Y[:, :, None]
+        inner = np.abs(Y - 0.5) < 0.25
+        rho = np.where(inner, 2.0, 1.0)
+        vx = np.where(inner, 0.5, -0.5)
+        vy = 0.01 * np.sin(4 * np.pi * X) + rng.uniform(-1e-3, 1e-3, n)
+        fields = [rho, rho * vx, rho * vy, np.zeros(n), 2.5 / 0.4 + 0.5 * rho * (vx * vx + vy * vy)]
+        params = {"heat_capacity_ratio": 1.4, "dynamic_viscosity": 1e-4,
+                  "gravitational_acceleration": [0.0, 0.0, 0.0]}
+        rows = [[1] * 6] * 5
+        lim = slope_limiters.minmod
+    else:
+        import bench
+        fields = bench.rt_fields(n[0])
+        params = bench.PARAMS
+        rows = bench.bc_rows()
+        lim = slope_limiters.van_leer
+    for k, f in zip(ns.VARS, fields):
+        U[k][1:-1, 1:-1, 1:-1] = torch.from_numpy(np.asarray(f, dtype=np.float64)).cuda()
+    rule = {0: bc.neumann, 1: bc.periodic, 2: bc.reflecting}
+    bcs = {k: {s: rule[c] for s, c in zip(bc.SIDES, row)} for k, row in zip(ns.VARS, rows)}
+    schemes = {"riemann_solver": riemann_solvers.hll, "reconstructor": reconstructors.muscl,
+               "slope_limiter": lim, "diffusive_solver": diffusive_solvers.centered}
+    return ns, grid, U, params, schemes, bcs
+
+
+def _errors(ref, got, names):
+    out = {}
+    for v, name in enumerate(names):
+        r = ref[v, 1:-1, 1:-1, 1:-1]
+        g = got[v, 1:-1, 1:-1, 1:-1]
+        d = (g - r).abs()
+        out[name] = (float(d.mean()), float(d.max()))
+    return out
+
+
+@pytest.mark.parametrize("case,n,steps", [("kh", (128, 128, 1), 200), ("rt", (32, 32, 32), 40)])
+def test_compensated_fp32_matches_fp64(case, n, steps):
+    from mfvm import _lib
+    ns, grid, U, params, schemes, bcs = _setup(n, case)
+    Uc = ns.to_compensated(U)
+    Up = ns.allocate_u({"nx1": n[0], "nx2": n[1], "nx3": n[2]}, dtype=torch.float32)
+    Up.buf.copy_(U.buf.to(torch.float32))
+    ns.advance(U, grid, params, schemes, bcs, _lib.INT_SSPRK3, 0.4, steps)
+    ns.advance_compensated(Uc, grid, params, schemes, bcs, _lib.INT_SSPRK3, 0.4, steps)
+    ns.advance(Up, grid, params, schemes, bcs, _lib.INT_SSPRK3, 0.4, steps)
+    torch.cuda.synchronize()
+    comp = _errors(U.buf, ns.compensated_value(Uc), ns.VARS)
+    plain = _errors(U.buf, Up.buf.to(torch.float64), ns.VARS)
+    for name in ns.VARS:
+        print(f"{case} {name}: comp L1 {comp[name][0]:.3e} Linf {comp[name][1]:.3e} | "
+              f"plain fp32 L1 {plain[name][0]:.3e} Linf {plain[name][1]:.3e}", flush=True)
+    for name in ns.VARS:
+        assert comp[name][0] <= 1e-6, (name, comp[name])
