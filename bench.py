@@ -31,6 +31,11 @@ HBM_PEAK_GBS = 8000.0
 # write (120 B each).
 BYTES_PER_CELL_STEP = 320.0
 STAGES = 3
+# HBM bytes per stage launch of ns_stage_kernel at n=256 with the default tile,
+# from two rocprofv3 --pmc runs of this workload (FETCH_SIZE, WRITE_SIZE in
+# KiB per dispatch) scaled by what the same counters read for a 1 GiB device
+# copy (bytes read = FETCH_SIZE * 1024 * 2.0, written = WRITE_SIZE * 1024 * 1.0).
+MEASURED_STAGE_TRAFFIC = {256: (2458328 * 2.0 + 696341 * 1.0) * 1024}
 
 
 def rt_config(n):
@@ -151,6 +156,7 @@ def main():
     stage_ms = kms[_lib.TK_STAGE] / max(kcnt[_lib.TK_STAGE], 1)
     bytes_per_launch = BYTES_PER_CELL_STEP / STAGES * cells
     achieved = bytes_per_launch / (stage_ms / 1e3) / 1e9
+    traffic = MEASURED_STAGE_TRAFFIC.get(n) if tile is None else None
     out = {
         "metric": "Mcell-updates/s", "value": round(value, 2), "unit": "Mcell-updates/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -160,7 +166,7 @@ def main():
                                "SSPRK3, mu=1e-4, g=-0.1 x3, periodic x1/x2, reflecting x3",
                    "cells": cells, "integrator": "SSPRK3", "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "ns_stage_kernel", "avg_launch_ms": round(stage_ms, 4),
                      "bytes_per_launch": bytes_per_launch},
         "step_hbm_frac": round(value * 1e6 * BYTES_PER_CELL_STEP / (HBM_PEAK_GBS * 1e9), 4),

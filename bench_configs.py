@@ -1,0 +1,186 @@
+"""Per-configuration measurements beside bench.py (one JSON line per config).
+
+  kh128    2D Kelvin-Helmholtz 128x128x1, MUSCL+minmod, HLL, SSPRK3, CFL 0.4,
+           200 steps, FP64 (with the NumPy oracle's wall time for the same run
+           unless --no-cpu).
+  maxwell  Maxwell pulse on a spherical grid 128x128x256 (r in [0.1,1],
+           theta in [0.05, pi-0.05], phi in [0, 2pi)), MUSCL+van Leer, Rusanov,
+           RK4, 300 steps, FP64 and FP32 (error of FP32 vs FP64 reported).
+  heat     heat diffusion on a cylindrical grid 256x128x128, kappa=1e-2, hot
+           disc T=2 in T=1, piecewise-constant + centred flux, Euler, 300 steps,
+           FP64 and FP32.
+
+Usage: python bench_configs.py [kh128|maxwell|heat|all] [--steps N] [--no-cpu]
+"""
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "multiphysics-fvm_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import numpy as np  # noqa: E402
+
+
+def _cfg(n, lo, hi):
+    return {"x1_min": lo[0], "x1_max": hi[0], "nx1": n[0], "x2_min": lo[1], "x2_max": hi[1],
+            "nx2": n[1], "x3_min": lo[2], "x3_max": hi[2], "nx3": n[2]}
+
+
+def _time(fn, torch):
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1)
+
+
+def kh128(args, torch):
+    from mfvm import _lib
+    from mfvm.grid import boundary_conditions as bc
+    from mfvm.grid.geometry import build_grid
+    from mfvm.numerics import diffusive_solvers, reconstructors, riemann_solvers, slope_limiters
+    from mfvm.physics import navier_stokes as ns
+    n = (128, 128, 1)
+    cfg = _cfg(n, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+    rng = np.random.default_rng(0)
+    x = (np.arange(128) + 0.5) / 128
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    X, Y = X[:, :, None], Y[:, :, None]
+    inner = np.abs(Y - 0.5) < 0.25
+    rho = np.where(inner, 2.0, 1.0)
+    vx = np.where(inner, 0.5, -0.5)
+    vy = 0.01 * np.sin(4 * np.pi * X) + rng.uniform(-1e-3, 1e-3, n)
+    fields = [rho, rho * vx, rho * vy, np.zeros(n), 2.5 / 0.4 + 0.5 * rho * (vx * vx + vy * vy)]
+    params = {"heat_capacity_ratio": 1.4, "dynamic_viscosity": 1e-4,
+              "gravitational_acceleration": [0.0, 0.0, 0.0]}
+    grid = build_grid("cartesian", cfg)
+    U = ns.allocate_u(cfg)
+    for k, f in zip(ns.VARS, fields):
+        U[k][1:-1, 1:-1, 1:-1] = torch.from_numpy(f).cuda()
+    bcs = {k: {s: bc.periodic for s in bc.SIDES} for k in ns.VARS}
+    schemes = {"riemann_solver": riemann_solvers.hll, "reconstructor": reconstructors.muscl,
+               "slope_limiter": slope_limiters.minmod, "diffusive_solver": diffusive_solvers.centered}
+    steps = args.steps or 200
+    ns.advance(U, grid, params, schemes, bcs, _lib.INT_SSPRK3, 0.4, 2)  # warm-up on a copy below
+    for k, f in zip(ns.VARS, fields):
+        U[k][1:-1, 1:-1, 1:-1] = torch.from_numpy(f).cuda()
+    ms = _time(lambda: ns.advance(U, grid, params, schemes, bcs, _lib.INT_SSPRK3, 0.4, steps), torch)
+    out = {"config": "kh128", "steps": steps, "ms": round(ms, 3),
+           "Mcell_updates_per_s": round(128 * 128 * steps / (ms / 1e3) / 1e6, 2), "dtype": "f64"}
+    if not args.no_cpu:
+        from oracle import grid as og
+        from oracle import navier_stokes as ons
+        from oracle import numerics as on
+        from oracle import solver as osv
+        Uo = ons.allocate_u(cfg)
+        for k, f in zip(ons.VARS, fields):
+            Uo[k][1:-1, 1:-1, 1:-1] = f
+        bo = {k: {s: og.periodic for s in og.SIDES} for k in ons.VARS}
+        og.apply_bcs(Uo, bo)
+        so = {"riemann_solver": on.hll, "reconstructor": on.muscl, "slope_limiter": on.minmod,
+              "diffusive_solver": on.centered}
+        t0 = time.perf_counter()
+        osv.advance(ons, Uo, og.build_grid("cartesian", cfg), params, so, on.rk3_ssp, bo, 0.4, steps)
+        wall = time.perf_counter() - t0
+        same = all(np.array_equal(U[k].cpu().numpy(), Uo[k]) for k in ns.VARS)
+        out.update({"cpu_oracle_s": round(wall, 2), "cpu_cores": 1, "bitwise_equal_to_oracle": same,
+                    "speedup_vs_cpu": round(wall * 1e3 / ms, 1)})
+    return out
+
+
+def maxwell(args, torch):
+    from mfvm import _lib
+    from mfvm.grid import boundary_conditions as bc
+    from mfvm.grid.geometry import build_grid
+    from mfvm.numerics import diffusive_solvers, reconstructors, riemann_solvers, slope_limiters
+    from mfvm.physics import maxwell as mx
+    n = (128, 128, 256)
+    cfg = _cfg(n, (0.1, 0.05, 0.0), (1.0, math.pi - 0.05, 2 * math.pi))
+    grid = build_grid("spherical", cfg)
+    params = {"magnetic_permeability": 1.0, "electric_permittivity": 1.0,
+              "electrical_conductivity": 0.0, "charge_density": 0.0,
+              "magnetic_cleaning_coeff": 1e-3, "electric_cleaning_coeff": 1e-3}
+    r = 0.1 + (np.arange(n[0]) + 0.5) * (0.9 / n[0])
+    th = 0.05 + (np.arange(n[1]) + 0.5) * ((math.pi - 0.1) / n[1])
+    R, TH = np.meshgrid(r, th, indexing="ij")
+    Ez = np.exp(-((R - 0.5) / 0.05) ** 2)
+    er = (Ez * np.cos(TH))[:, :, None] * np.ones(n)
+    eth = (-Ez * np.sin(TH))[:, :, None] * np.ones(n)
+    schemes = {"riemann_solver": riemann_solvers.rusanov, "reconstructor": reconstructors.muscl,
+               "slope_limiter": slope_limiters.van_leer, "diffusive_solver": diffusive_solvers.centered}
+    bcs = {k: {"left": bc.neumann, "right": bc.neumann, "bottom": bc.neumann, "top": bc.neumann,
+               "front": bc.periodic, "back": bc.periodic} for k in mx.VARS}
+    steps = args.steps or 300
+    res = {}
+    for dt_name, dtype in (("f64", torch.float64), ("f32", torch.float32)):
+        U = mx.allocate_u(cfg, dtype=dtype)
+        U["electric_d_field_x1"][1:-1, 1:-1, 1:-1] = torch.from_numpy(er).to(dtype).cuda()
+        U["electric_d_field_x2"][1:-1, 1:-1, 1:-1] = torch.from_numpy(eth).to(dtype).cuda()
+        ms = _time(lambda: mx.advance(U, grid, params, schemes, bcs, _lib.INT_RK4, 0.4, steps), torch)
+        res[dt_name] = (ms, U.buf.to(torch.float64))
+    d = (res["f32"][1] - res["f64"][1])[:, 1:-1, 1:-1, 1:-1].abs()
+    cells = n[0] * n[1] * n[2]
+    return {"config": "maxwell_spherical_128x128x256_rk4", "steps": steps,
+            "f64_ms": round(res["f64"][0], 2), "f32_ms": round(res["f32"][0], 2),
+            "f64_Mcell_updates_per_s": round(cells * steps / res["f64"][0] * 1e-3, 1),
+            "f32_Mcell_updates_per_s": round(cells * steps / res["f32"][0] * 1e-3, 1),
+            "f32_vs_f64_L1": float(d.mean()), "f32_vs_f64_Linf": float(d.max())}
+
+
+def heat(args, torch):
+    from mfvm import _lib
+    from mfvm.grid import boundary_conditions as bc
+    from mfvm.grid.geometry import build_grid
+    from mfvm.numerics import diffusive_solvers, reconstructors, riemann_solvers, slope_limiters
+    from mfvm.physics import heat as ht
+    n = (256, 128, 128)
+    cfg = _cfg(n, (0.0, 0.0, 0.0), (1.0, 2 * math.pi, 1.0))
+    grid = build_grid("cylindrical", cfg)
+    params = {"mass_density": 1.0, "specific_heat_capacity": 1.0, "thermal_conductivity": 1e-2,
+              "volumetric_heat_source": 0.0}
+    r = (np.arange(n[0]) + 0.5) / n[0]
+    T0 = np.where(r < 0.25, 2.0, 1.0)[:, None, None] * np.ones(n)
+    schemes = {"riemann_solver": riemann_solvers.hll, "reconstructor": reconstructors.piecewise_constant,
+               "slope_limiter": slope_limiters.minmod, "diffusive_solver": diffusive_solvers.centered}
+    bcs = {"internal_energy_density": {"left": bc.neumann, "right": bc.neumann, "bottom": bc.periodic,
+                                       "top": bc.periodic, "front": bc.neumann, "back": bc.neumann}}
+    steps = args.steps or 300
+    res = {}
+    for dt_name, dtype in (("f64", torch.float64), ("f32", torch.float32)):
+        U = ht.allocate_u(cfg, dtype=dtype)
+        U["internal_energy_density"][1:-1, 1:-1, 1:-1] = torch.from_numpy(T0).to(dtype).cuda()
+        ms = _time(lambda: ht.advance(U, grid, params, schemes, bcs, _lib.INT_EULER, 0.4, steps), torch)
+        res[dt_name] = (ms, U.buf.to(torch.float64))
+    d = (res["f32"][1] - res["f64"][1])[:, 1:-1, 1:-1, 1:-1].abs()
+    cells = n[0] * n[1] * n[2]
+    return {"config": "heat_cylindrical_256x128x128_euler", "steps": steps,
+            "f64_ms": round(res["f64"][0], 2), "f32_ms": round(res["f32"][0], 2),
+            "f64_Mcell_updates_per_s": round(cells * steps / res["f64"][0] * 1e-3, 1),
+            "f32_Mcell_updates_per_s": round(cells * steps / res["f32"][0] * 1e-3, 1),
+            "f64_hbm_frac_16B": round(cells * steps * 16 / (res["f64"][0] * 1e-3) / 8e12, 4),
+            "f32_vs_f64_L1": float(d.mean()), "f32_vs_f64_Linf": float(d.max())}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("which", nargs="?", default="all")
+    ap.add_argument("--steps", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    import torch
+    todo = ["kh128", "maxwell", "heat"] if args.which == "all" else [args.which]
+    for name in todo:
+        print(json.dumps(globals()[name](args, torch)), flush=True)
+
+
+if __name__ == "__main__":
+    main()
