@@ -49,14 +49,14 @@ int t_begin(int kind, hipStream_t s) {
   t.a = take_event();
   t.b = take_event();
   if (!t.a || !t.b) return -1;
-  hipEventRecord(t.a, s);
+  (void)hipEventRecord(t.a, s);
   g_launches.push_back(t);
   return (int)g_launches.size() - 1;
 }
 
 void t_end(int token, hipStream_t s) {
   if (token < 0 || token >= (int)g_launches.size()) return;
-  hipEventRecord(g_launches[token].b, s);
+  (void)hipEventRecord(g_launches[token].b, s);
 }
 
 static void t_recycle() {
@@ -146,7 +146,7 @@ int32_t mfvm_abi_version(void) { return 1; }
 int mfvm_timing(int32_t mode, double* ms, int64_t* count) {
   if (mode == 0 || mode == 1) {
     if (!g_launches.empty()) {
-      hipDeviceSynchronize();
+      (void)hipDeviceSynchronize();
       t_recycle();
     }
     g_timing = mode == 1;

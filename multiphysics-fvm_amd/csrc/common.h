@@ -99,6 +99,34 @@ template <class R> MFVM_DEV R div_len(const Geo& g, int d, R x, R len) {
   return g.lpow2[d] ? x * R(g.linv[d]) : x / len;
 }
 
+// Tile coordinates of this workgroup.  The dispatcher hands workgroups to the
+// 8 XCDs round-robin by linear id, and each XCD has its own L2.  With
+// `remap`, the linear id is renumbered so that XCD x runs one contiguous range
+// of tiles (x1 fastest, then x2, then x3): tiles whose halo rows overlap then
+// run on the same XCD at the same time and share those rows through its L2.
+// The map is a bijection on [0, nblocks) for any grid size.
+struct TileId {
+  int x, y, z;
+};
+MFVM_DEV TileId tile_id(bool remap) {
+  TileId t;
+  if (!remap) {
+    t.x = blockIdx.x;
+    t.y = blockIdx.y;
+    t.z = blockIdx.z;
+    return t;
+  }
+  const unsigned gx = gridDim.x, gy = gridDim.y;
+  const unsigned nb = gx * gy * gridDim.z;
+  const unsigned w = (blockIdx.z * gy + blockIdx.y) * gx + blockIdx.x;
+  const unsigned q = nb >> 3, r = nb & 7u, x = w & 7u;
+  const unsigned L = x * q + min(x, r) + (w >> 3);
+  t.x = L % gx;
+  t.y = (L / gx) % gy;
+  t.z = L / (gx * gy);
+  return t;
+}
+
 // Ghosted SoA addressing: field-major, x1 fastest.
 struct Layout {
   int n1, n2, n3;        // interior sizes

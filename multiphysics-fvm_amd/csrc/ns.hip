@@ -52,6 +52,7 @@ struct StageArgs {
   const double* dt_dev;
   double* partials;
   int kz;
+  int remap;  // XCD-aware tile order (tile_id)
   // compensated FP32 stages (MFVM_COMP_*): lo = compensation of U0 (ghosted),
   // dinc = increment over U0 (interior), snext = next stage state (ghosted)
   void* lo;
@@ -63,44 +64,40 @@ struct StageArgs {
 // ---------------------------------------------------------------------------
 // Closure (navier_stokes.py:44-156): P on interior + face ghosts, T interior.
 // ---------------------------------------------------------------------------
-template <bool UNIF, class R>
-__global__ void ns_closure_kernel(Layout lay, Geo geo, NSPar p, const R* __restrict__ U,
-                                  R* __restrict__ C) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = blockIdx.z;
-  const int n1 = lay.n1, n2 = lay.n2, n3 = lay.n3;
-  if (i > n1 + 1 || j > n2 + 1) return;
-  const int nb = (i == 0 || i == n1 + 1) + (j == 0 || j == n2 + 1) + (k == 0 || k == n3 + 1);
-  if (nb > 1) return;  // ghost edges and corners are never read by the RHS
-  const long long fs = lay.fs;
-  const long long c = lay.at(i, j, k);
-  const R rho = U[c];
-  const R v1 = U[c + fs] / rho, v2 = U[c + 2 * fs] / rho, v3 = U[c + 3 * fs] / rho;
-  const R E = U[c + 4 * fs] / rho;
-  // :100  P = (ga-1)*rho*(E - (v1**2+v2**2+v3**2)/2)
-  C[c] = (R(p.gm1) * rho) * (E - (((v1 * v1) + (v2 * v2)) + (v3 * v3)) / R(2));
-  if (nb != 0 || !p.visc) return;
+// :100  P = (ga-1)*rho*(E - (v1**2+v2**2+v3**2)/2), from the cell's U.
+template <class R>
+MFVM_DEV R closure_p(const NSPar& p, R rho, R v1, R v2, R v3, R rhoE) {
+  const R E = rhoE / rho;
+  return (R(p.gm1) * rho) * (E - (((v1 * v1) + (v2 * v2)) + (v3 * v3)) / R(2));
+}
 
-  // :111-119 centred gradients g[a][b] = d v_a / d x_b over 2*L_b
-  const long long st[3] = {1, lay.sy, lay.sz};
+// :94-97 primitive velocity of the cell at offset o.
+template <class R>
+MFVM_DEV void load_v(const R* __restrict__ U, long long o, long long fs, R& rho, R v[3]) {
+  rho = U[o];
+  v[0] = U[o + fs] / rho;
+  v[1] = U[o + 2 * fs] / rho;
+  v[2] = U[o + 3 * fs] / rho;
+}
+
+// :111-139 Stokes tensor from the centred differences dv[a][b] = v_a(+b) -
+// v_a(-b) and the cell's own velocity; writes the six unique entries.
+template <bool UNIF, class R>
+MFVM_DEV void closure_t(const Geo& geo, const NSPar& p, int i, int j, int k, const R dv[3][3],
+                        const R v[3], R* __restrict__ C, long long c, long long fs) {
   R g[3][3];
 #pragma unroll
   for (int b = 0; b < 3; ++b) {
-    const long long op = c + st[b], om = c - st[b];
-    const R rp = U[op], rm = U[om];
-    const R vp[3] = {U[op + fs] / rp, U[op + 2 * fs] / rp, U[op + 3 * fs] / rp};
-    const R vm[3] = {U[om + fs] / rm, U[om + 2 * fs] / rm, U[om + 3 * fs] / rm};
-    R twoL;
     if (geo.lpow2[b]) {
 #pragma unroll
-      for (int a = 0; a < 3; ++a) g[a][b] = (vp[a] - vm[a]) * R(0.5 * geo.linv[b]);
+      for (int a = 0; a < 3; ++a) g[a][b] = dv[a][b] * R(0.5 * geo.linv[b]);
     } else {
-      twoL = R(2) * g_len<UNIF, R>(geo, b, i, j, k);
+      const R twoL = R(2) * g_len<UNIF, R>(geo, b, i, j, k);
 #pragma unroll
-      for (int a = 0; a < 3; ++a) g[a][b] = (vp[a] - vm[a]) / twoL;
+      for (int a = 0; a < 3; ++a) g[a][b] = dv[a][b] / twoL;
     }
   }
+  const R v1 = v[0], v2 = v[1], v3 = v[2];
   const R mu = R(p.mu);
   const R two3 = R(2.0 / 3.0);
   R t11, t22, t33, t12, t13, t23;
@@ -138,6 +135,117 @@ __global__ void ns_closure_kernel(Layout lay, Geo geo, NSPar p, const R* __restr
   C[c + C_T12 * fs] = t12;
   C[c + C_T13 * fs] = t13;
   C[c + C_T23 * fs] = t23;
+}
+
+// Inviscid closure: P on the interior and the face ghosts (edges and corners
+// are never read by the RHS).  T stays +-0 and is not written.
+template <class R>
+__global__ void ns_closure_p_kernel(Layout lay, NSPar p, const R* __restrict__ U, R* __restrict__ C) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  const int n1 = lay.n1, n2 = lay.n2, n3 = lay.n3;
+  if (i > n1 + 1 || j > n2 + 1) return;
+  const int nb = (i == 0 || i == n1 + 1) + (j == 0 || j == n2 + 1) + (k == 0 || k == n3 + 1);
+  if (nb > 1) return;
+  const long long fs = lay.fs, c = lay.at(i, j, k);
+  R rho, v[3];
+  load_v(U, c, fs, rho, v);
+  C[c] = closure_p(p, rho, v[0], v[1], v[2], U[c + 4 * fs]);
+}
+
+// Viscous closure, marching in x3.  One thread per (x1, x2) column of a
+// BX x BY tile over a chunk of kz planes.  Each velocity is computed once:
+// the column keeps v at k-1, k, k+1 in registers, and plane k of the tile
+// plus a one-cell ring is exchanged through LDS for the x1 and x2
+// differences.  The ring cells are computed by helper threads; where a ring
+// cell is a face ghost of the domain, its helper also writes its P.
+template <bool UNIF, class R>
+__global__ __launch_bounds__(256) void ns_closure_visc_kernel(Layout lay, Geo geo, NSPar p,
+                                                              const R* __restrict__ U,
+                                                              R* __restrict__ C, int kz, int remap) {
+  extern __shared__ double smem_raw[];
+  R* vs = reinterpret_cast<R*>(smem_raw);  // [3][BY+2][BX+2]
+  const int BX = blockDim.x, BY = blockDim.y;
+  const int sx = BX + 2, splane = (BY + 2) * sx;
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * BX + tx;
+  const int n1 = lay.n1, n2 = lay.n2, n3 = lay.n3;
+  const long long fs = lay.fs;
+  const TileId tl = tile_id(remap != 0);
+  const int i0 = 1 + tl.x * BX, j0 = 1 + tl.y * BY;
+  const int kb = 1 + tl.z * kz, ke = min(kb + kz, n3 + 1);
+  const int ilast = min(i0 + BX - 1, n1), jlast = min(j0 + BY - 1, n2);
+  const int i = i0 + tx, j = j0 + ty;
+  const bool own = i <= n1 && j <= n2;
+
+  // Ring cell of this helper thread (h < 2*BY + 2*BX), in global and LDS
+  // coordinates; hv = false when the ring slot lies beyond the tile's cells.
+  int hi = 0, hj = 0, hli = 0, hlj = 0;
+  bool hv = false;
+  {
+    const int h = tid;
+    if (h < BY) {
+      hj = j0 + h; hi = i0 - 1; hli = 0; hlj = h + 1; hv = hj <= jlast;
+    } else if (h < 2 * BY) {
+      hj = j0 + h - BY; hi = ilast + 1; hli = hi - i0 + 1; hlj = h - BY + 1; hv = hj <= jlast;
+    } else if (h < 2 * BY + BX) {
+      hi = i0 + h - 2 * BY; hj = j0 - 1; hli = hi - i0 + 1; hlj = 0; hv = hi <= ilast;
+    } else if (h < 2 * BY + 2 * BX) {
+      hi = i0 + h - 2 * BY - BX; hj = jlast + 1; hli = hi - i0 + 1; hlj = hj - j0 + 1; hv = hi <= ilast;
+    }
+  }
+  const bool hghost = hv && (hi == 0 || hi == n1 + 1 || hj == 0 || hj == n2 + 1);
+
+  R vm[3] = {R(0), R(0), R(0)}, vc[3] = {R(0), R(0), R(0)}, vn[3];
+  if (own) {
+    R rho;
+    const long long om = lay.at(i, j, kb - 1), oc = lay.at(i, j, kb);
+    load_v(U, om, fs, rho, vm);
+    if (kb - 1 == 0) C[om] = closure_p(p, rho, vm[0], vm[1], vm[2], U[om + 4 * fs]);
+    load_v(U, oc, fs, rho, vc);
+    C[oc] = closure_p(p, rho, vc[0], vc[1], vc[2], U[oc + 4 * fs]);
+  }
+  for (int k = kb; k < ke; ++k) {
+    __syncthreads();  // the previous plane's LDS reads are done
+    if (own) {
+      const int l = (ty + 1) * sx + (tx + 1);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) vs[a * splane + l] = vc[a];
+    }
+    if (hv) {
+      const long long oh = lay.at(hi, hj, k);
+      R rho, vh[3];
+      load_v(U, oh, fs, rho, vh);
+      const int l = hlj * sx + hli;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) vs[a * splane + l] = vh[a];
+      if (hghost) C[oh] = closure_p(p, rho, vh[0], vh[1], vh[2], U[oh + 4 * fs]);
+    }
+    if (own) {
+      const long long on = lay.at(i, j, k + 1);
+      R rho;
+      load_v(U, on, fs, rho, vn);
+      if (k + 1 < ke || k + 1 == n3 + 1) C[on] = closure_p(p, rho, vn[0], vn[1], vn[2], U[on + 4 * fs]);
+    }
+    __syncthreads();
+    if (own) {
+      const int l = (ty + 1) * sx + (tx + 1);
+      R dv[3][3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const R* va = vs + a * splane;
+        dv[a][0] = va[l + 1] - va[l - 1];
+        dv[a][1] = va[l + sx] - va[l - sx];
+        dv[a][2] = vn[a] - vm[a];
+      }
+      closure_t<UNIF, R>(geo, p, i, j, k, dv, vc, C, lay.at(i, j, k), fs);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        vm[a] = vc[a];
+        vc[a] = vn[a];
+      }
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -346,7 +454,7 @@ MFVM_DEV void cell_cfl(const Geo& g, const NSPar& p, int i, int j, int k, const 
 
 // Block max of (rate, nu) -> partials[2*block], NaN-propagating.
 template <class R>
-MFVM_DEV void block_max2(R rate, R nu, double* partials, double* sred) {
+MFVM_DEV void block_max2(R rate, R nu, double* slot, double* sred) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
     rate = nmax(rate, __shfl_xor(rate, o, 64));
@@ -366,29 +474,55 @@ MFVM_DEV void block_max2(R rate, R nu, double* partials, double* sred) {
       a = nmax(a, sred[2 * w]);
       b = nmax(b, sred[2 * w + 1]);
     }
-    const long long blk = (long long)(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-    partials[2 * blk] = a;
-    partials[2 * blk + 1] = b;
+    slot[0] = a;
+    slot[1] = b;
   }
+}
+
+// Linear id of this workgroup in its launch grid.
+MFVM_DEV long long hw_block_id() {
+  return (long long)(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
 }
 
 // ---------------------------------------------------------------------------
 // Fused stage kernel
 // ---------------------------------------------------------------------------
+
+// The tile one workgroup advances: tile coordinates (x1, x2 tile and x3
+// chunk), tile shape (bx * by threads take part, bx fastest), and where the
+// tile's CFL maxima go (NULL: no CFL reduction).
+struct TileCtx {
+  int x, y, z;
+  int bx, by;
+  double* slot;
+};
+
+// Tile of a one-workgroup-per-tile launch over one grid.
+MFVM_DEV TileCtx grid_tile(const StageArgs& a) {
+  const TileId t = tile_id(a.remap != 0);
+  TileCtx c;
+  c.x = t.x;
+  c.y = t.y;
+  c.z = t.z;
+  c.bx = blockDim.x;
+  c.by = blockDim.y;
+  c.slot = a.partials ? a.partials + 2 * hw_block_id() : nullptr;
+  return c;
+}
 template <int LIM, int RS, bool UNIF, class R, bool COMP>
-MFVM_DEV void ns_stage_body(const StageArgs& a) {
+MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl) {
   extern __shared__ double smem_raw[];
   R* sm = reinterpret_cast<R*>(smem_raw);
   const R* __restrict__ U = static_cast<const R*>(a.U);
   const R* __restrict__ C = static_cast<const R*>(a.C);
-  const int BX = blockDim.x, BY = blockDim.y;
-  const int tx = threadIdx.x, ty = threadIdx.y;
-  const int tid = ty * BX + tx;
+  const int BX = tl.bx, BY = tl.by;
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  const int tx = tid % BX, ty = tid / BX;
   const int n1 = a.lay.n1, n2 = a.lay.n2, n3 = a.lay.n3;
   const long long fs = a.lay.fs;
-  const int i0 = 1 + blockIdx.x * BX, j0 = 1 + blockIdx.y * BY;
+  const int i0 = 1 + tl.x * BX, j0 = 1 + tl.y * BY;
   const int i = i0 + tx, j = j0 + ty;
-  const int kb = 1 + blockIdx.z * a.kz;
+  const int kb = 1 + tl.z * a.kz;
   const int ke = min(kb + a.kz, n3 + 1);
   const bool own = (i <= n1) && (j <= n2);
   const bool visc = a.p.visc != 0;
@@ -720,18 +854,18 @@ MFVM_DEV void ns_stage_body(const StageArgs& a) {
     }
   }
 
-  if (a.partials) block_max2<double>(rate_max, nu_max, a.partials, reinterpret_cast<double*>(smem_raw));
+  if (tl.slot) block_max2<double>(rate_max, nu_max, tl.slot, reinterpret_cast<double*>(smem_raw));
 }
 
 template <int LIM, int RS, bool UNIF, class R>
 __global__ __launch_bounds__(256) void ns_stage_kernel(StageArgs a) {
-  ns_stage_body<LIM, RS, UNIF, R, false>(a);
+  ns_stage_body<LIM, RS, UNIF, R, false>(a, grid_tile(a));
 }
 
 // Compensated FP32 stage: half the registers of the FP64 kernel, two waves per SIMD.
 template <int LIM, int RS, bool UNIF>
 __global__ __launch_bounds__(256, 2) void ns_stage_comp_kernel(StageArgs a) {
-  ns_stage_body<LIM, RS, UNIF, float, true>(a);
+  ns_stage_body<LIM, RS, UNIF, float, true>(a, grid_tile(a));
 }
 
 // Standalone CFL partials (navier_stokes.py:512-583), same grid as the stage.
@@ -798,6 +932,15 @@ struct Tile {
   int bx, by, kz;
 };
 
+// XCD-aware tile order (tile_id); MFVM_XCD_REMAP=0 selects the plain order.
+static int xcd_remap_enabled() {
+  static const int on = [] {
+    const char* e = getenv("MFVM_XCD_REMAP");
+    return e ? atoi(e) : 1;
+  }();
+  return on;
+}
+
 Tile ns_tile(const mfvm_grid* g, const mfvm_ns_params* p) {
   Tile t;
   t.bx = p->tile[0] > 0 ? p->tile[0] : 64;
@@ -835,15 +978,23 @@ hipError_t launch_closure(const mfvm_grid* g, const mfvm_ns_params* p, const voi
   const Layout lay = make_layout(g->n[0], g->n[1], g->n[2]);
   const Geo geo = make_geo(g);
   const NSPar q = make_ns_par(p);
+  const R* u = static_cast<const R*>(U);
+  R* c = static_cast<R*>(C);
   const dim3 blk(64, 4, 1);
-  const dim3 grd((g->n[0] + 2 + 63) / 64, (g->n[1] + 2 + 3) / 4, g->n[2] + 2);
   const int tk = t_begin(MFVM_TK_CLOSURE, s);
-  if (g->uniform)
-    hipLaunchKernelGGL((ns_closure_kernel<true, R>), grd, blk, 0, s, lay, geo, q, static_cast<const R*>(U),
-                       static_cast<R*>(C));
-  else
-    hipLaunchKernelGGL((ns_closure_kernel<false, R>), grd, blk, 0, s, lay, geo, q, static_cast<const R*>(U),
-                       static_cast<R*>(C));
+  if (!q.visc) {
+    const dim3 grd((g->n[0] + 2 + 63) / 64, (g->n[1] + 2 + 3) / 4, g->n[2] + 2);
+    hipLaunchKernelGGL((ns_closure_p_kernel<R>), grd, blk, 0, s, lay, q, u, c);
+  } else {
+    const int kz = g->n[2] < 32 ? g->n[2] : 32;
+    const dim3 grd((g->n[0] + 63) / 64, (g->n[1] + 3) / 4, (g->n[2] + kz - 1) / kz);
+    const size_t lds = 3 * (4 + 2) * (64 + 2) * sizeof(R);
+    const int remap = xcd_remap_enabled();
+    if (g->uniform)
+      hipLaunchKernelGGL((ns_closure_visc_kernel<true, R>), grd, blk, lds, s, lay, geo, q, u, c, kz, remap);
+    else
+      hipLaunchKernelGGL((ns_closure_visc_kernel<false, R>), grd, blk, lds, s, lay, geo, q, u, c, kz, remap);
+  }
   t_end(tk, s);
   return hipGetLastError();
 }
@@ -887,6 +1038,7 @@ hipError_t launch_stage(const mfvm_grid* g, const mfvm_ns_params* p, const mfvm_
   a.dt_dev = st->dt_dev;
   a.partials = st->partials;
   a.kz = t.kz;
+  a.remap = xcd_remap_enabled();
   a.lo = st->lo;
   a.dinc = st->dinc;
   a.snext = st->snext;
