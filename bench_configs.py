@@ -9,8 +9,14 @@
   heat     heat diffusion on a cylindrical grid 256x128x128, kappa=1e-2, hot
            disc T=2 in T=1, piecewise-constant + centred flux, Euler, 300 steps,
            FP64 and FP32.
+  ensemble 64 Kelvin-Helmholtz instances: 56 2D grids with nx1, nx2 drawn from
+           {64,128,256,512} (seed 2) and eight 64^3 grids; per instance (seed
+           3+i) shear U[0.3,0.7], density ratio U[1.5,3], phase U[0,2pi);
+           MUSCL+MC, Rusanov, RK3, mu=1e-4, periodic, 500 steps; autotuned
+           launch shape, graph replay; FP64 and FP32-compensated, against
+           advancing the members one by one.
 
-Usage: python bench_configs.py [kh128|maxwell|heat|all] [--steps N] [--no-cpu]
+Usage: python bench_configs.py [kh128|maxwell|heat|ensemble|all] [--steps N] [--no-cpu]
 """
 
 import argparse
@@ -170,6 +176,93 @@ def heat(args, torch):
             "f32_vs_f64_L1": float(d.mean()), "f32_vs_f64_Linf": float(d.max())}
 
 
+def ensemble_members(seed=2):
+    """Sizes of the 64-member Kelvin-Helmholtz ensemble."""
+    rng = np.random.default_rng(seed)
+    sizes = rng.choice([64, 128, 256, 512], size=(56, 2))
+    return [(int(a), int(b), 1) for a, b in sizes] + [(64, 64, 64)] * 8
+
+
+def kh_instance(n, seed):
+    """Kelvin-Helmholtz instance: shear U[0.3,0.7], density ratio U[1.5,3],
+    perturbation phase U[0,2pi), 1e-3 velocity noise (vz only in 3D)."""
+    rng = np.random.default_rng(seed)
+    shear = rng.uniform(0.3, 0.7)
+    ratio = rng.uniform(1.5, 3.0)
+    phase = rng.uniform(0.0, 2 * math.pi)
+    x = (np.arange(n[0]) + 0.5) / n[0]
+    y = (np.arange(n[1]) + 0.5) / n[1]
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    X, Y = X[:, :, None], Y[:, :, None]
+    inner = np.abs(Y - 0.5) < 0.25
+    rho = np.where(inner, ratio, 1.0) * np.ones(n)
+    vx = np.where(inner, shear, -shear) * np.ones(n)
+    vy = 0.01 * np.sin(4 * np.pi * X + phase) + rng.uniform(-1e-3, 1e-3, n)
+    vz = rng.uniform(-1e-3, 1e-3, n) if n[2] > 1 else np.zeros(n)
+    E = 2.5 / 0.4 + 0.5 * rho * (vx * vx + vy * vy + vz * vz)
+    return [rho, rho * vx, rho * vy, rho * vz, E]
+
+
+def ensemble(args, torch):
+    from mfvm.ensemble import Ensemble
+    from mfvm.grid import boundary_conditions as bc
+    from mfvm.grid.geometry import build_grid
+    from mfvm.numerics import diffusive_solvers, reconstructors, riemann_solvers, slope_limiters
+    from mfvm.numerics import time_integrators as ti
+    from mfvm.physics import navier_stokes as ns
+    shapes = ensemble_members()
+    params = {"heat_capacity_ratio": 1.4, "dynamic_viscosity": 1e-4,
+              "gravitational_acceleration": [0.0, 0.0, 0.0]}
+    schemes = {"riemann_solver": riemann_solvers.rusanov, "reconstructor": reconstructors.muscl,
+               "slope_limiter": slope_limiters.monotonized_central,
+               "diffusive_solver": diffusive_solvers.centered}
+    bcs = {k: {s: bc.periodic for s in bc.SIDES} for k in ns.VARS}
+    steps = args.steps or 500
+    grids, fields = [], []
+    for i, n in enumerate(shapes):
+        grids.append(build_grid("cartesian", _cfg(n, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0))))
+        fields.append(kh_instance(n, 3 + i))
+
+    def states():
+        out = []
+        for n, f in zip(shapes, fields):
+            U = ns.allocate_u({"nx1": n[0], "nx2": n[1], "nx3": n[2]})
+            for k, a in zip(ns.VARS, f):
+                U[k][1:-1, 1:-1, 1:-1] = torch.from_numpy(np.ascontiguousarray(a)).cuda()
+            out.append(U)
+        return out
+
+    cells = sum(a * b * c for a, b, c in shapes)
+    out = {"config": "ensemble_kh64_rk3", "members": len(shapes), "cells": cells, "steps": steps}
+    for leg in ("f64", "f32_compensated"):
+        comp = leg != "f64"
+        S = states()
+        if comp:
+            S = [ns.to_compensated(U) for U in S]
+        ens = Ensemble(S, grids, params, schemes, bcs, ti.rk3, compensated=comp)
+        t0 = time.perf_counter()
+        best = ens.autotune(0.4, steps=3)
+        tune_s = time.perf_counter() - t0
+        ms = _time(lambda: ens.advance(0.4, steps), torch)
+        out[leg] = {"ms": round(ms, 2), "Mcell_updates_per_s": round(cells * steps / ms * 1e-3, 1),
+                    "autotune_s": round(tune_s, 2), "trials": len(ens.tuning_log),
+                    "options": {"order": best["order"], "persist": best["persist"],
+                                "remap": best["remap"], "graph": best["graph"],
+                                "tiles": {"x".join(map(str, k)): list(v) for k, v in best["tiles"].items()}}}
+        del ens, S
+    # the same members advanced one at a time (single-grid launches), on a
+    # bounded number of steps
+    seq_steps = min(steps, 20)
+    S = states()
+    rk3 = ti.rk3._mfvm_code
+    ms = _time(lambda: [ns.advance(U, g, params, schemes, bcs, rk3, 0.4, seq_steps)
+                        for U, g in zip(S, grids)], torch)
+    seq = cells * seq_steps / ms * 1e-3
+    out["one_by_one"] = {"steps": seq_steps, "ms": round(ms, 2), "Mcell_updates_per_s": round(seq, 1)}
+    out["batching_speedup_f64"] = round(out["f64"]["Mcell_updates_per_s"] / seq, 2)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("which", nargs="?", default="all")
@@ -177,7 +270,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     import torch
-    todo = ["kh128", "maxwell", "heat"] if args.which == "all" else [args.which]
+    todo = ["kh128", "maxwell", "heat", "ensemble"] if args.which == "all" else [args.which]
     for name in todo:
         print(json.dumps(globals()[name](args, torch)), flush=True)
 

@@ -276,6 +276,58 @@ typedef struct mfvm_ns_comp_work {
 int mfvm_ns_comp_advance(const mfvm_grid* g, const mfvm_ns_params* p, const mfvm_ns_comp_work* w,
                          int32_t integrator, double cfl, int32_t nsteps, void* stream);
 
+/* ---- batched ensemble of Navier-Stokes instances ---------------------------
+ * No counterpart in the reference, which advances one grid per process
+ * (src/solver.py:88-151); each member here advances exactly as
+ * mfvm_ns_advance / mfvm_ns_comp_advance would advance it alone (same
+ * arithmetic, its own CFL dt every step), while every phase of a stage (ghost
+ * fill, closure, fused stage) is one launch over the tiles of all members,
+ * and the steps can be replayed from captured hipGraphs. */
+
+#define MFVM_ENS_ORDER_MEMBER 0      /* work list: member by member */
+#define MFVM_ENS_ORDER_COST 1        /* costliest tiles first (x3 planes per tile, then size) */
+#define MFVM_ENS_ORDER_INTERLEAVE 2  /* round-robin over the members */
+
+typedef struct mfvm_ens_member {
+  mfvm_grid grid;               /* sizes and metric factors of this member */
+  int32_t bc[5][6];             /* ghost rule per field and side */
+  int32_t tile[3];              /* stage tile x1, x2 threads (product 256) and x3 planes;
+                                   0 = 64, 4, min(nx3, 32) */
+  void* U;                      /* ghosted state, advanced in place (compensated: hi, f32) */
+  void* S1;                     /* ghosted stage states (compensated: Sa, Sb) */
+  void* S2;
+  void* C;                      /* ghosted closure, 7 fields; ghost entries must be 0 */
+  void* kacc;                   /* interior accumulator, 5 fields */
+  void* lo;                     /* compensated only: ghosted f32 compensation words */
+  void* dinc;                   /* compensated only: interior f32 increment */
+  double* partials;             /* >= 2 * mfvm_ens_member_tiles() doubles */
+  double* dts;                  /* device, nsteps+1 entries: dt of every step */
+} mfvm_ens_member;
+
+typedef struct mfvm_ens_opts {
+  int32_t order;                /* MFVM_ENS_ORDER_* */
+  int32_t persist;              /* 0: one workgroup per tile; k > 0: k workgroups per CU,
+                                   each looping over the work list */
+  int32_t graph;                /* > 0: steps per captured hipGraph (stream must not be the
+                                   null stream); 0: direct launches */
+  int32_t remap;                /* XCD-aware work order */
+} mfvm_ens_opts;
+
+/* Stage tiles of one member (its partials need twice as many doubles). */
+int64_t mfvm_ens_member_tiles(const mfvm_ens_member* m);
+/* Bytes of the device table mfvm_ens_advance needs for these members. */
+int64_t mfvm_ens_table_bytes(const mfvm_ens_member* m, int32_t n, int32_t integrator);
+/* nsteps steps of every member.  p gives the shared scheme and physics
+ * (dtype, limiter, riemann, gamma, mu, g; its bc and tile are not used).
+ * compensated != 0: FP32 fluxes with FP64-compensated accumulation (dtype
+ * f32).  table: caller-owned device buffer of mfvm_ens_table_bytes().
+ * Euler is not supported (MFVM_E_UNSUP).  Synchronises the stream before
+ * returning when graph > 0. */
+int mfvm_ens_advance(const mfvm_ens_member* m, int32_t n, const mfvm_ns_params* p,
+                     const mfvm_ens_opts* o, int32_t integrator, int32_t compensated,
+                     double cfl, int32_t nsteps, void* table, int64_t table_bytes,
+                     void* stream);
+
 /* ---- heat equation (src/physics/heat.py) -------------------------------------- */
 
 /* Right-hand side into an interior buffer.  Replaces heat.py:354-473. */

@@ -19,6 +19,12 @@ hipError_t heat_launch_dt(const mfvm_grid*, const mfvm_heat_params*, double, dou
 template <class R> hipError_t maxwell_launch_res(const mfvm_grid*, const mfvm_maxwell_params*, const void*, void*, hipStream_t);
 template <class R> hipError_t maxwell_launch_stage(const mfvm_grid*, const mfvm_maxwell_params*, const void*, const void*, const StageIO&, hipStream_t);
 hipError_t maxwell_launch_dt(const mfvm_grid*, const mfvm_maxwell_params*, double, double*, hipStream_t);
+// Defined in ensemble.hip.
+int ens_nstages(int integrator);
+long long ens_member_tiles(const mfvm_ens_member* m);
+long long ens_table_bytes(const mfvm_ens_member* m, int n, int integrator);
+hipError_t ens_advance(const mfvm_ens_member* m, int n, const mfvm_ns_params* p, const mfvm_ens_opts* o,
+                       int integrator, bool comp, double cfl, int nsteps, void* table, hipStream_t s);
 
 static thread_local std::string g_err;
 
@@ -484,6 +490,64 @@ int mfvm_ns_comp_advance(const mfvm_grid* g, const mfvm_ns_params* p, const mfvm
   }
   if (ns > 1 && (rc = mfvm_apply_bcs(g, 5, bc, MFVM_F32, seq[ns - 1].in, w->hi, stream))) return rc;
   return MFVM_OK;
+}
+
+// ---- batched ensemble ----
+static int check_ens_members(const mfvm_ens_member* m, int32_t n) {
+  if (!m || n < 1) return fail(MFVM_E_ARG, "ensemble: need at least one member");
+  for (int i = 0; i < n; ++i) {
+    int rc = check_grid(&m[i].grid);
+    if (!rc) rc = check_bc(5, &m[i].bc[0][0]);
+    if (rc) return rc;
+    const int bx = m[i].tile[0] > 0 ? m[i].tile[0] : 64, by = m[i].tile[1] > 0 ? m[i].tile[1] : 4;
+    if (bx < 2 || by < 1 || bx * by != 256 || m[i].tile[2] < 0)
+      return fail(MFVM_E_ARG, "ensemble: member tile must have x1 threads >= 2, x1*x2 threads == 256, x3 planes >= 0");
+  }
+  return MFVM_OK;
+}
+
+int64_t mfvm_ens_member_tiles(const mfvm_ens_member* m) {
+  if (check_ens_members(m, 1)) return -1;
+  return ens_member_tiles(m);
+}
+
+int64_t mfvm_ens_table_bytes(const mfvm_ens_member* m, int32_t n, int32_t integrator) {
+  if (check_ens_members(m, n)) return -1;
+  if (ens_nstages(integrator) == 0) {
+    fail(MFVM_E_UNSUP, "ensemble: integrator must be rk3, rk3_ssp or rk4");
+    return -1;
+  }
+  return ens_table_bytes(m, n, integrator);
+}
+
+int mfvm_ens_advance(const mfvm_ens_member* m, int32_t n, const mfvm_ns_params* p, const mfvm_ens_opts* o,
+                     int32_t integrator, int32_t compensated, double cfl, int32_t nsteps, void* table,
+                     int64_t table_bytes, void* stream) {
+  int rc = check_ens_members(m, n);
+  if (rc) return rc;
+  if (!p || !o) return fail(MFVM_E_ARG, "ensemble: NULL params or options");
+  if (p->limiter < MFVM_LIM_NONE || p->limiter > MFVM_LIM_VANLEER)
+    return fail(MFVM_E_UNSUP, "unsupported slope limiter / reconstructor");
+  if (p->riemann != MFVM_RS_HLL && p->riemann != MFVM_RS_RUSANOV)
+    return fail(MFVM_E_UNSUP, "unsupported Riemann solver");
+  if (ens_nstages(integrator) == 0) return fail(MFVM_E_UNSUP, "ensemble: integrator must be rk3, rk3_ssp or rk4");
+  if (compensated && p->dtype != MFVM_F32) return fail(MFVM_E_ARG, "ensemble: compensated runs need dtype f32");
+  if (!compensated && p->dtype != MFVM_F64) return fail(MFVM_E_ARG, "ensemble: plain runs need dtype f64");
+  if (o->order < MFVM_ENS_ORDER_MEMBER || o->order > MFVM_ENS_ORDER_INTERLEAVE || o->persist < 0 || o->graph < 0)
+    return fail(MFVM_E_ARG, "ensemble: bad options");
+  for (int i = 0; i < n; ++i) {
+    const mfvm_ens_member& x = m[i];
+    if (!x.U || !x.S1 || !x.S2 || !x.C || !x.kacc || !x.partials || !x.dts)
+      return fail(MFVM_E_ARG, "ensemble: NULL member buffer");
+    if (compensated && (!x.lo || !x.dinc)) return fail(MFVM_E_ARG, "ensemble: compensated member needs lo and dinc");
+  }
+  if (nsteps < 0) return fail(MFVM_E_ARG, "nsteps must be >= 0");
+  const long long need = ens_table_bytes(m, n, integrator);
+  if (!table || table_bytes < need) return fail(MFVM_E_ARG, "ensemble: table too small");
+  if (nsteps == 0) return MFVM_OK;
+  return hip_check(ens_advance(m, n, p, o, integrator, compensated != 0, cfl, nsteps, table,
+                               static_cast<hipStream_t>(stream)),
+                   "ens_advance");
 }
 
 // ---- heat ----

@@ -1,0 +1,606 @@
+// Batched ensemble of Navier-Stokes instances (mfvm_ens_advance).
+//
+// Every member advances with exactly the arithmetic of the single-grid path
+// (the same device bodies: closure_march_body, ns_stage_body, cfl_tile_body,
+// shell_value), so each member's result equals its own mfvm_ns_advance /
+// mfvm_ns_comp_advance run bit for bit.  What changes is the launch shape:
+// each phase of a stage (ghost fill, closure, fused stage) is ONE launch over
+// a work list holding the tiles of all members, so a step costs the same
+// number of launches for 64 members as for one, and small members no longer
+// leave the GPU idle.  The work list order, the stage tile of each member and
+// persistent vs one-workgroup-per-tile launches are options (picked by the
+// autotuner in mfvm.ensemble); the steps can be captured into a hipGraph and
+// replayed.
+//
+// Device table (caller-owned, mfvm_ens_table_bytes):
+//   EnsMem[n] | StageArgs[nstages][n] | stage work | closure work | dt[n] | step[n]
+// dt[m] is the member's current dt; step[m] counts the dts written to
+// m.dts (the finalize kernel appends one per step), so one captured step is
+// valid for every replay.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "bc_core.h"
+#include "ns_core.h"
+
+namespace mfvm {
+
+struct EnsMem {
+  Layout lay;
+  ILayout il;
+  Geo geo;
+  BCArgs bca;
+  int bx, by, kz;           // stage tile
+  int ntx, nty, ntz;        // stage tiles per axis
+  int cnx, cny, cnz, ckz;   // closure tiles (64 x 4 x ckz)
+  long long nshell;         // ghost cells per field
+  long long bc_off;         // first ghost-fill thread of this member
+  long long ntiles;         // stage tiles
+  void* buf[3];             // stage states: U (hi), S1 (Sa), S2 (Sb)
+  void* C;
+  const void* lo;           // compensated: compensation words of U
+  double* partials;
+  double* dts;
+  double* dt_cur;
+  int* step;
+  double smax;
+};
+
+struct EnsWork {
+  int m, t;  // member, tile of the member
+};
+
+constexpr int kEnsBlock = 256;
+
+// Work item of list index v.  With remap, XCD x (which receives the indices
+// v = x mod 8, see tile_id) gets one contiguous range of the list.
+MFVM_DEV long long ens_item(long long v, long long n, bool remap) {
+  if (!remap) return v;
+  const unsigned long long q = (unsigned long long)n >> 3, r = (unsigned long long)n & 7ull;
+  const unsigned long long x = (unsigned long long)v & 7ull;
+  return (long long)(x * q + (x < r ? x : r) + ((unsigned long long)v >> 3));
+}
+
+template <class R>
+__global__ __launch_bounds__(kEnsBlock) void ens_bc_kernel(const EnsMem* __restrict__ mem, int nm, int src_sel,
+                                                           int dst_sel, long long total) {
+  const long long t = (long long)blockIdx.x * kEnsBlock + threadIdx.x;
+  if (t >= total) return;
+  int lo = 0, hi = nm - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (mem[mid].bc_off <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  const EnsMem& e = mem[lo];
+  const long long u = t - e.bc_off;
+  const int var = (int)(u / e.nshell);
+  int i, j, k;
+  shell_cell(e.lay, u % e.nshell, i, j, k);
+  const R* src = static_cast<const R*>(e.buf[src_sel]) + var * e.lay.fs;
+  R* dst = static_cast<R*>(e.buf[dst_sel]);
+  dst[var * e.lay.fs + e.lay.at(i, j, k)] = shell_value<R>(e.bca, var, src, i, j, k);
+}
+
+template <bool VISC, bool UNIF, class R>
+__global__ __launch_bounds__(kEnsBlock) void ens_closure_kernel(const EnsMem* __restrict__ mem, NSPar p,
+                                                                const EnsWork* __restrict__ work,
+                                                                long long nwork, int sel, int remap) {
+  for (long long v = blockIdx.x; v < nwork; v += gridDim.x) {
+    const EnsWork w = work[ens_item(v, nwork, remap != 0)];
+    const EnsMem& e = mem[w.m];
+    TileId tl;
+    tl.x = w.t % e.cnx;
+    tl.y = (w.t / e.cnx) % e.cny;
+    tl.z = w.t / (e.cnx * e.cny);
+    closure_march_body<VISC, UNIF, R>(e.lay, e.geo, p, static_cast<const R*>(e.buf[sel]),
+                                      static_cast<R*>(e.C), e.ckz, tl);
+    __syncthreads();  // LDS of this tile is read before the next tile writes it
+  }
+}
+
+template <int LIM, int RS, bool UNIF, class R, bool COMP>
+MFVM_DEV void ens_stage_loop(const EnsMem* __restrict__ mem, const StageArgs* __restrict__ st,
+                             const EnsWork* __restrict__ work, long long nwork, int remap) {
+  for (long long v = blockIdx.x; v < nwork; v += gridDim.x) {
+    const EnsWork w = work[ens_item(v, nwork, remap != 0)];
+    const EnsMem& e = mem[w.m];
+    const StageArgs& a = st[w.m];
+    TileCtx tc;
+    tc.bx = e.bx;
+    tc.by = e.by;
+    tc.x = w.t % e.ntx;
+    tc.y = (w.t / e.ntx) % e.nty;
+    tc.z = w.t / (e.ntx * e.nty);
+    tc.slot = a.partials ? a.partials + 2 * (long long)w.t : nullptr;
+    ns_stage_body<LIM, RS, UNIF, R, COMP>(a, tc);
+    __syncthreads();
+  }
+}
+
+template <int LIM, int RS, bool UNIF>
+__global__ __launch_bounds__(kEnsBlock) void ens_stage_kernel(const EnsMem* __restrict__ mem,
+                                                              const StageArgs* __restrict__ st,
+                                                              const EnsWork* __restrict__ work,
+                                                              long long nwork, int remap) {
+  ens_stage_loop<LIM, RS, UNIF, double, false>(mem, st, work, nwork, remap);
+}
+
+template <int LIM, int RS, bool UNIF>
+__global__ __launch_bounds__(kEnsBlock, 2) void ens_stage_comp_kernel(const EnsMem* __restrict__ mem,
+                                                                      const StageArgs* __restrict__ st,
+                                                                      const EnsWork* __restrict__ work,
+                                                                      long long nwork, int remap) {
+  ens_stage_loop<LIM, RS, UNIF, float, true>(mem, st, work, nwork, remap);
+}
+
+template <bool UNIF, class R>
+__global__ __launch_bounds__(kEnsBlock) void ens_cfl_kernel(const EnsMem* __restrict__ mem, NSPar p,
+                                                            const EnsWork* __restrict__ work,
+                                                            long long nwork, int remap) {
+  for (long long v = blockIdx.x; v < nwork; v += gridDim.x) {
+    const EnsWork w = work[ens_item(v, nwork, remap != 0)];
+    const EnsMem& e = mem[w.m];
+    TileCtx tc;
+    tc.bx = e.bx;
+    tc.by = e.by;
+    tc.x = w.t % e.ntx;
+    tc.y = (w.t / e.ntx) % e.nty;
+    tc.z = w.t / (e.ntx * e.nty);
+    tc.slot = e.partials + 2 * (long long)w.t;
+    cfl_tile_body<UNIF, R>(e.lay, e.geo, p, static_cast<const R*>(e.buf[0]), static_cast<const R*>(e.lo),
+                           e.kz, tc);
+    __syncthreads();
+  }
+}
+
+// One workgroup per member: dt of the member's next step from its partials.
+__global__ __launch_bounds__(kEnsBlock) void ens_finalize_kernel(const EnsMem* __restrict__ mem, double cfl) {
+  const EnsMem& e = mem[blockIdx.x];
+  const double dt = dt_reduce_block(e.partials, e.ntiles, cfl, e.smax);
+  if (threadIdx.x == 0) {
+    const int s = *e.step;
+    e.dts[s] = dt;
+    *e.dt_cur = dt;
+    *e.step = s + 1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+struct EnsSeq {
+  int in, out;  // buffer selectors (0 U/hi, 1 S1/Sa, 2 S2/Sb; out -1: none)
+  int update, acc, comp;
+  double c0, c1, ac;
+};
+
+// Stage sequences of time_integrators.py (as in mfvm_ns_step / mfvm_ns_comp_advance).
+static int ens_sequence(int integrator, bool comp, EnsSeq* s, int* last_in) {
+  if (!comp) {
+    switch (integrator) {
+      case MFVM_INT_RK3:  // time_integrators.py:87-103
+        s[0] = {0, 1, MFVM_UPD_BASE, MFVM_ACC_SET, MFVM_COMP_NONE, 0.5, 0, 0};
+        s[1] = {1, 2, MFVM_UPD_BASE, MFVM_ACC_ADD, MFVM_COMP_NONE, 1.0, 0, 4.0};
+        s[2] = {2, 0, MFVM_UPD_ACC6, MFVM_ACC_NONE, MFVM_COMP_NONE, 0, 0, 0};
+        *last_in = 2;
+        return 3;
+      case MFVM_INT_SSPRK3:  // :143-159
+        s[0] = {0, 1, MFVM_UPD_EULER, MFVM_ACC_NONE, MFVM_COMP_NONE, 0, 0, 0};
+        s[1] = {1, 2, MFVM_UPD_SSP, MFVM_ACC_NONE, MFVM_COMP_NONE, 0.75, 0.25, 0};
+        s[2] = {2, 0, MFVM_UPD_SSP, MFVM_ACC_NONE, MFVM_COMP_NONE, 1.0 / 3.0, 2.0 / 3.0, 0};
+        *last_in = 2;
+        return 3;
+      case MFVM_INT_RK4:  // :199-221
+        s[0] = {0, 1, MFVM_UPD_BASE, MFVM_ACC_SET, MFVM_COMP_NONE, 0.5, 0, 0};
+        s[1] = {1, 2, MFVM_UPD_BASE, MFVM_ACC_ADD, MFVM_COMP_NONE, 0.5, 0, 2.0};
+        s[2] = {2, 1, MFVM_UPD_BASE, MFVM_ACC_ADD, MFVM_COMP_NONE, 1.0, 0, 2.0};
+        s[3] = {1, 0, MFVM_UPD_ACC6, MFVM_ACC_NONE, MFVM_COMP_NONE, 0, 0, 0};
+        *last_in = 1;
+        return 4;
+      default:
+        return 0;
+    }
+  }
+  switch (integrator) {
+    case MFVM_INT_RK3:
+      s[0] = {0, 1, MFVM_UPD_NONE, MFVM_ACC_SET, MFVM_COMP_SET, 0.5, 0, 0};
+      s[1] = {1, 2, MFVM_UPD_NONE, MFVM_ACC_ADD, MFVM_COMP_SET, 1.0, 0, 4.0};
+      s[2] = {2, -1, MFVM_UPD_NONE, MFVM_ACC_NONE, MFVM_COMP_FINAL_ACC6, 0, 0, 0};
+      *last_in = 2;
+      return 3;
+    case MFVM_INT_SSPRK3:
+      s[0] = {0, 1, MFVM_UPD_NONE, MFVM_ACC_NONE, MFVM_COMP_SET, 1.0, 0, 0};
+      s[1] = {1, 2, MFVM_UPD_NONE, MFVM_ACC_NONE, MFVM_COMP_SSP, 0.25, 0, 0};
+      s[2] = {2, -1, MFVM_UPD_NONE, MFVM_ACC_NONE, MFVM_COMP_FINAL_SSP, 2.0 / 3.0, 0, 0};
+      *last_in = 2;
+      return 3;
+    case MFVM_INT_RK4:
+      s[0] = {0, 1, MFVM_UPD_NONE, MFVM_ACC_SET, MFVM_COMP_SET, 0.5, 0, 0};
+      s[1] = {1, 2, MFVM_UPD_NONE, MFVM_ACC_ADD, MFVM_COMP_SET, 0.5, 0, 2.0};
+      s[2] = {2, 1, MFVM_UPD_NONE, MFVM_ACC_ADD, MFVM_COMP_SET, 1.0, 0, 2.0};
+      s[3] = {1, -1, MFVM_UPD_NONE, MFVM_ACC_NONE, MFVM_COMP_FINAL_ACC6, 0, 0, 0};
+      *last_in = 1;
+      return 4;
+    default:
+      return 0;
+  }
+}
+
+int ens_nstages(int integrator) {
+  EnsSeq s[4];
+  int last;
+  return ens_sequence(integrator, false, s, &last);
+}
+
+static void member_tiles(const mfvm_ens_member& m, int* bx, int* by, int* kz, int nt[3]) {
+  const int* n = m.grid.n;
+  *bx = m.tile[0] > 0 ? m.tile[0] : 64;
+  *by = m.tile[1] > 0 ? m.tile[1] : 4;
+  *kz = m.tile[2] > 0 ? m.tile[2] : (n[2] < 32 ? n[2] : 32);
+  nt[0] = (n[0] + *bx - 1) / *bx;
+  nt[1] = (n[1] + *by - 1) / *by;
+  nt[2] = (n[2] + *kz - 1) / *kz;
+}
+
+long long ens_member_tiles(const mfvm_ens_member* m) {
+  int bx, by, kz, nt[3];
+  member_tiles(*m, &bx, &by, &kz, nt);
+  return (long long)nt[0] * nt[1] * nt[2];
+}
+
+static void closure_tiles(const mfvm_ens_member& m, int nt[3], int* kz) {
+  const int* n = m.grid.n;
+  *kz = n[2] < 32 ? n[2] : 32;
+  nt[0] = (n[0] + 63) / 64;
+  nt[1] = (n[1] + 3) / 4;
+  nt[2] = (n[2] + *kz - 1) / *kz;
+}
+
+struct EnsLayout {
+  size_t off_mem, off_st, off_sw, off_cw, off_dt, off_step, bytes;
+  long long nsw, ncw;
+};
+
+static EnsLayout ens_layout(const mfvm_ens_member* m, int n, int nst) {
+  EnsLayout L;
+  L.nsw = 0;
+  L.ncw = 0;
+  for (int i = 0; i < n; ++i) {
+    L.nsw += ens_member_tiles(&m[i]);
+    int nt[3], kz;
+    closure_tiles(m[i], nt, &kz);
+    L.ncw += (long long)nt[0] * nt[1] * nt[2];
+  }
+  size_t o = 0;
+  L.off_mem = o;
+  o = align256(o + sizeof(EnsMem) * n);
+  L.off_st = o;
+  o = align256(o + sizeof(StageArgs) * n * nst);
+  L.off_sw = o;
+  o = align256(o + sizeof(EnsWork) * L.nsw);
+  L.off_cw = o;
+  o = align256(o + sizeof(EnsWork) * L.ncw);
+  L.off_dt = o;
+  o = align256(o + sizeof(double) * n);
+  L.off_step = o;
+  o = align256(o + sizeof(int) * n);
+  L.bytes = o;
+  return L;
+}
+
+long long ens_table_bytes(const mfvm_ens_member* m, int n, int integrator) {
+  const int nst = ens_nstages(integrator);
+  if (nst == 0) return -1;
+  return (long long)ens_layout(m, n, nst).bytes;
+}
+
+// Work list of the members' tiles in the requested order.
+static void ens_worklist(const std::vector<long long>& ntiles, const std::vector<double>& cost, int order,
+                         std::vector<EnsWork>& out) {
+  const int n = (int)ntiles.size();
+  out.clear();
+  std::vector<int> ord(n);
+  for (int i = 0; i < n; ++i) ord[i] = i;
+  if (order == MFVM_ENS_ORDER_COST)
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
+      return cost[a] != cost[b] ? cost[a] > cost[b] : ntiles[a] > ntiles[b];
+    });
+  if (order == MFVM_ENS_ORDER_INTERLEAVE) {
+    long long mx = 0;
+    for (long long t : ntiles) mx = std::max(mx, t);
+    for (long long t = 0; t < mx; ++t)
+      for (int i = 0; i < n; ++i)
+        if (t < ntiles[i]) out.push_back({i, (int)t});
+    return;
+  }
+  for (int i : ord)
+    for (long long t = 0; t < ntiles[i]; ++t) out.push_back({i, (int)t});
+}
+
+struct EnsRun {
+  int n, nst, last_in;
+  bool comp, unif, visc;
+  int lim, rs;
+  NSPar par;
+  double cfl;
+  EnsMem* d_mem;
+  StageArgs* d_st;
+  EnsWork* d_sw;
+  EnsWork* d_cw;
+  long long nsw, ncw, bc_total;
+  size_t lds_stage;
+  int persist_blocks;  // 0: one workgroup per item
+  int remap;
+  int in_sel[4];
+};
+
+static unsigned grid_for(long long nitems, int persist_blocks) {
+  if (persist_blocks > 0 && nitems > persist_blocks) return (unsigned)persist_blocks;
+  return (unsigned)nitems;
+}
+
+template <int LIM, int RS, bool UNIF>
+static void launch_ens_stage_t(const EnsRun& r, const StageArgs* st, hipStream_t s) {
+  const unsigned g = grid_for(r.nsw, r.persist_blocks);
+  if (r.comp)
+    hipLaunchKernelGGL((ens_stage_comp_kernel<LIM, RS, UNIF>), dim3(g), dim3(kEnsBlock), r.lds_stage, s, r.d_mem,
+                       st, r.d_sw, r.nsw, r.remap);
+  else
+    hipLaunchKernelGGL((ens_stage_kernel<LIM, RS, UNIF>), dim3(g), dim3(kEnsBlock), r.lds_stage, s, r.d_mem, st,
+                       r.d_sw, r.nsw, r.remap);
+}
+
+template <int LIM, int RS>
+static void launch_ens_stage_u(const EnsRun& r, const StageArgs* st, hipStream_t s) {
+  if (r.unif) launch_ens_stage_t<LIM, RS, true>(r, st, s);
+  else launch_ens_stage_t<LIM, RS, false>(r, st, s);
+}
+
+template <int LIM>
+static void launch_ens_stage_l(const EnsRun& r, const StageArgs* st, hipStream_t s) {
+  if (r.rs == MFVM_RS_HLL) launch_ens_stage_u<LIM, MFVM_RS_HLL>(r, st, s);
+  else launch_ens_stage_u<LIM, MFVM_RS_RUSANOV>(r, st, s);
+}
+
+static void launch_ens_stage(const EnsRun& r, int q, hipStream_t s) {
+  const StageArgs* st = r.d_st + (size_t)q * r.n;
+  switch (r.lim) {
+    case MFVM_LIM_NONE: launch_ens_stage_l<MFVM_LIM_NONE>(r, st, s); break;
+    case MFVM_LIM_MINMOD: launch_ens_stage_l<MFVM_LIM_MINMOD>(r, st, s); break;
+    case MFVM_LIM_MC: launch_ens_stage_l<MFVM_LIM_MC>(r, st, s); break;
+    default: launch_ens_stage_l<MFVM_LIM_VANLEER>(r, st, s); break;
+  }
+}
+
+template <class R>
+static void launch_ens_bc(const EnsRun& r, int src, int dst, hipStream_t s) {
+  const unsigned g = (unsigned)((r.bc_total + kEnsBlock - 1) / kEnsBlock);
+  hipLaunchKernelGGL(ens_bc_kernel<R>, dim3(g), dim3(kEnsBlock), 0, s, r.d_mem, r.n, src, dst, r.bc_total);
+}
+
+template <class R>
+static void launch_ens_closure(const EnsRun& r, int sel, hipStream_t s) {
+  const unsigned g = grid_for(r.ncw, r.persist_blocks);
+  const size_t lds = 3 * (4 + 2) * (64 + 2) * sizeof(R);
+  const dim3 blk(64, 4, 1);
+  if (r.visc) {
+    if (r.unif)
+      hipLaunchKernelGGL((ens_closure_kernel<true, true, R>), dim3(g), blk, lds, s, r.d_mem, r.par, r.d_cw, r.ncw,
+                         sel, r.remap);
+    else
+      hipLaunchKernelGGL((ens_closure_kernel<true, false, R>), dim3(g), blk, lds, s, r.d_mem, r.par, r.d_cw, r.ncw,
+                         sel, r.remap);
+  } else {
+    hipLaunchKernelGGL((ens_closure_kernel<false, true, R>), dim3(g), blk, 0, s, r.d_mem, r.par, r.d_cw, r.ncw,
+                       sel, r.remap);
+  }
+}
+
+template <class R>
+static void launch_ens_cfl(const EnsRun& r, hipStream_t s) {
+  const unsigned g = grid_for(r.nsw, r.persist_blocks);
+  const size_t red = 2 * sizeof(double) * (kEnsBlock / 64);
+  if (r.unif)
+    hipLaunchKernelGGL((ens_cfl_kernel<true, R>), dim3(g), dim3(kEnsBlock), red, s, r.d_mem, r.par, r.d_sw, r.nsw,
+                       r.remap);
+  else
+    hipLaunchKernelGGL((ens_cfl_kernel<false, R>), dim3(g), dim3(kEnsBlock), red, s, r.d_mem, r.par, r.d_sw, r.nsw,
+                       r.remap);
+  hipLaunchKernelGGL(ens_finalize_kernel, dim3(r.n), dim3(kEnsBlock), 0, s, r.d_mem, r.cfl);
+}
+
+// One step of every member: per stage a ghost fill, the closure and the
+// fused stage; then each member's dt of the next step.
+static hipError_t ens_step(const EnsRun& r, hipStream_t s) {
+  for (int q = 0; q < r.nst; ++q) {
+    const int in = r.in_sel[q];
+    if (r.comp) {
+      launch_ens_bc<float>(r, in, in, s);
+      launch_ens_closure<float>(r, in, s);
+    } else {
+      launch_ens_bc<double>(r, in, in, s);
+      launch_ens_closure<double>(r, in, s);
+    }
+    launch_ens_stage(r, q, s);
+  }
+  hipLaunchKernelGGL(ens_finalize_kernel, dim3(r.n), dim3(kEnsBlock), 0, s, r.d_mem, r.cfl);
+  return hipGetLastError();
+}
+
+// Builds the device table (synchronous upload) and runs nsteps.  Arguments
+// are checked by the caller (mfvm_ens_advance).
+hipError_t ens_advance(const mfvm_ens_member* m, int n, const mfvm_ns_params* p, const mfvm_ens_opts* o,
+                       int integrator, bool comp, double cfl, int nsteps, void* table, hipStream_t s) {
+  EnsSeq seq[4];
+  int last_in = 0;
+  const int nst = ens_sequence(integrator, comp, seq, &last_in);
+  const EnsLayout L = ens_layout(m, n, nst);
+  char* base = static_cast<char*>(table);
+
+  EnsRun r;
+  r.n = n;
+  r.nst = nst;
+  r.last_in = last_in;
+  r.comp = comp;
+  r.lim = p->limiter;
+  r.rs = p->riemann;
+  r.par = make_ns_par(p);
+  r.visc = r.par.visc != 0;
+  r.cfl = cfl;
+  r.unif = true;
+  for (int i = 0; i < n; ++i) r.unif = r.unif && m[i].grid.uniform;
+  r.d_mem = reinterpret_cast<EnsMem*>(base + L.off_mem);
+  r.d_st = reinterpret_cast<StageArgs*>(base + L.off_st);
+  r.d_sw = reinterpret_cast<EnsWork*>(base + L.off_sw);
+  r.d_cw = reinterpret_cast<EnsWork*>(base + L.off_cw);
+  r.nsw = L.nsw;
+  r.ncw = L.ncw;
+  r.remap = o->remap ? 1 : 0;
+  for (int q = 0; q < nst; ++q) r.in_sel[q] = seq[q].in;
+  double* d_dt = reinterpret_cast<double*>(base + L.off_dt);
+  int* d_step = reinterpret_cast<int*>(base + L.off_step);
+
+  const size_t rsize = comp ? sizeof(float) : sizeof(double);
+  std::vector<char> host(L.bytes, 0);
+  EnsMem* hm = reinterpret_cast<EnsMem*>(host.data() + L.off_mem);
+  StageArgs* hs = reinterpret_cast<StageArgs*>(host.data() + L.off_st);
+  std::vector<long long> st_tiles(n), cl_tiles(n);
+  std::vector<double> st_cost(n), cl_cost(n);
+  long long bc_off = 0;
+  size_t lds = 0;
+  for (int i = 0; i < n; ++i) {
+    const mfvm_ens_member& mm = m[i];
+    EnsMem e;
+    e.lay = make_layout(mm.grid.n[0], mm.grid.n[1], mm.grid.n[2]);
+    e.il = make_ilayout(mm.grid.n[0], mm.grid.n[1], mm.grid.n[2]);
+    e.geo = make_geo(&mm.grid);
+    e.bca.lay = e.lay;
+    e.bca.nvar = 5;
+    for (int v = 0; v < 5; ++v)
+      for (int d = 0; d < 6; ++d) e.bca.bc[v][d] = mm.bc[v][d];
+    int nt[3];
+    member_tiles(mm, &e.bx, &e.by, &e.kz, nt);
+    e.ntx = nt[0];
+    e.nty = nt[1];
+    e.ntz = nt[2];
+    int ct[3];
+    closure_tiles(mm, ct, &e.ckz);
+    e.cnx = ct[0];
+    e.cny = ct[1];
+    e.cnz = ct[2];
+    e.nshell = shell_count(mm.grid.n[0], mm.grid.n[1], mm.grid.n[2]);
+    e.bc_off = bc_off;
+    bc_off += 5 * e.nshell;
+    e.ntiles = (long long)nt[0] * nt[1] * nt[2];
+    e.buf[0] = mm.U;
+    e.buf[1] = mm.S1;
+    e.buf[2] = mm.S2;
+    e.C = mm.C;
+    e.lo = comp ? mm.lo : nullptr;
+    e.partials = mm.partials;
+    e.dts = mm.dts;
+    e.dt_cur = d_dt + i;
+    e.step = d_step + i;
+    e.smax = mm.grid.smax;
+    hm[i] = e;
+    st_tiles[i] = e.ntiles;
+    st_cost[i] = e.kz;
+    cl_tiles[i] = (long long)ct[0] * ct[1] * ct[2];
+    cl_cost[i] = e.ckz;
+    lds = std::max(lds, stage_lds_bytes(e.bx, e.by, rsize));
+    for (int q = 0; q < nst; ++q) {
+      StageArgs a;
+      memset(&a, 0, sizeof(a));
+      a.lay = e.lay;
+      a.il = e.il;
+      a.geo = e.geo;
+      a.p = r.par;
+      a.U = e.buf[seq[q].in];
+      a.C = mm.C;
+      a.U0 = mm.U;
+      a.kacc = mm.kacc;
+      a.update = seq[q].update;
+      a.acc = seq[q].acc;
+      a.c0 = seq[q].c0;
+      a.c1 = seq[q].c1;
+      a.acc_coef = seq[q].ac;
+      a.dt = 0.0;
+      a.dt_dev = e.dt_cur;
+      a.partials = (q == nst - 1) ? mm.partials : nullptr;
+      a.kz = e.kz;
+      a.remap = 0;
+      a.comp = seq[q].comp;
+      if (comp) {
+        const bool fin = seq[q].comp >= MFVM_COMP_FINAL_SET;
+        a.out = fin ? mm.U : nullptr;
+        a.lo = mm.lo;
+        a.dinc = mm.dinc;
+        a.snext = fin ? nullptr : e.buf[seq[q].out];
+      } else {
+        a.out = e.buf[seq[q].out];
+        a.lo = nullptr;
+        a.dinc = nullptr;
+        a.snext = nullptr;
+      }
+      hs[(size_t)q * n + i] = a;
+    }
+  }
+  r.bc_total = bc_off;
+  r.lds_stage = lds;
+  std::vector<EnsWork> sw, cw;
+  ens_worklist(st_tiles, st_cost, o->order, sw);
+  ens_worklist(cl_tiles, cl_cost, o->order, cw);
+  memcpy(host.data() + L.off_sw, sw.data(), sizeof(EnsWork) * sw.size());
+  memcpy(host.data() + L.off_cw, cw.data(), sizeof(EnsWork) * cw.size());
+
+  int dev = 0, ncu = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e != hipSuccess) return e;
+  r.persist_blocks = o->persist > 0 ? ((o->persist * ncu + 7) / 8) * 8 : 0;
+
+  // Upload (the host copy must outlive the transfer: synchronous).
+  if ((e = hipMemcpyAsync(table, host.data(), L.bytes, hipMemcpyHostToDevice, s)) != hipSuccess) return e;
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+
+  if (comp) launch_ens_cfl<float>(r, s);
+  else launch_ens_cfl<double>(r, s);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+
+  int done = 0;
+  if (o->graph > 0 && nsteps > 0) {
+    const int chunk = std::min(o->graph, nsteps);
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    if ((e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal)) != hipSuccess) return e;
+    hipError_t ce = hipSuccess;
+    for (int c = 0; c < chunk && ce == hipSuccess; ++c) ce = ens_step(r, s);
+    e = hipStreamEndCapture(s, &graph);
+    if (ce != hipSuccess) e = ce;
+    if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    for (int rep = 0; e == hipSuccess && rep < nsteps / chunk; ++rep) {
+      e = hipGraphLaunch(exec, s);
+      done += chunk;
+    }
+    const hipError_t se = hipStreamSynchronize(s);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return e;
+    if (se != hipSuccess) return se;
+  }
+  for (; done < nsteps; ++done)
+    if ((e = ens_step(r, s)) != hipSuccess) return e;
+  // Leave the ghosts as the reference does: the fill of the last stage input.
+  if (nsteps > 0) {
+    if (comp) launch_ens_bc<float>(r, last_in, 0, s);
+    else launch_ens_bc<double>(r, last_in, 0, s);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace mfvm

@@ -1,0 +1,965 @@
+// Compressible Navier-Stokes device code shared by the single-grid kernels
+// (ns.hip) and the batched ensemble kernels (ensemble.hip): closure,
+// reconstruction, fluxes, sources, the fused RK stage body and the CFL
+// reduction.
+//
+// Reference: src/physics/navier_stokes.py and src/numerics/* of
+// multiphysics-fvm.  Every expression keeps the reference operation order
+// (see the line citations), which makes the float64 instantiation bitwise
+// equal to the NumPy reference.
+//
+// Stage kernel design (gfx950):
+//  * one thread per (x1, x2) column of a BX x BY tile; the block marches in
+//    x3 over a chunk of planes, keeping the x3 stencil (three planes of the
+//    reconstruction fields, the carried left face state, the previous x3
+//    face flux and the in-plane partial divergence) in registers;
+//  * per plane, each thread reconstructs its own cell along x1 and x2 once
+//    (limited slopes shared by both faces of the cell); right-face states are
+//    exchanged through LDS, fluxes F.S are computed once per face and
+//    exchanged through LDS again for the divergence;
+//  * tile-edge cells (the left/right neighbours of the tile in x1 and x2)
+//    are reconstructed by helper threads, and their faces computed once;
+//  * the stage update (Euler / RK3 / SSPRK3 / RK4 forms) is fused into the
+//    epilogue of each cell, and the last stage of a step also reduces the CFL
+//    rates of the new state (wavefront shuffles, then LDS, then one partial
+//    per block).
+#pragma once
+
+#include <cstdlib>
+
+#include "common.h"
+
+namespace mfvm {
+
+
+enum { C_P = 0, C_T11 = 1, C_T22 = 2, C_T33 = 3, C_T12 = 4, C_T13 = 5, C_T23 = 6 };
+#define NS_NC 7   // closure fields
+#define NQ 9      // reconstruction fields per direction: U(5), P, row d of T
+
+// Closure field holding T[d][e] (symmetric: T21 is T12 etc., navier_stokes.py:137-139).
+MFVM_DEV constexpr int trow(int d, int e) {
+  return d == e ? (d == 0 ? C_T11 : (d == 1 ? C_T22 : C_T33))
+                : ((d + e == 1) ? C_T12 : ((d + e == 2) ? C_T13 : C_T23));
+}
+
+struct NSPar {
+  double gamma, gm1, mu, g[3];
+  int visc;  // mu != 0: viscous fluxes and closure (T == +-0 otherwise)
+  int grav;  // any g != 0: gravity source (+-0 otherwise)
+};
+
+struct StageArgs {
+  Layout lay;
+  ILayout il;
+  Geo geo;
+  NSPar p;
+  const void* U;
+  const void* C;
+  const void* U0;
+  void* out;
+  void* kacc;
+  int update, acc;
+  double c0, c1, acc_coef, dt;
+  const double* dt_dev;
+  double* partials;
+  int kz;
+  int remap;  // XCD-aware tile order (tile_id)
+  // compensated FP32 stages (MFVM_COMP_*): lo = compensation of U0 (ghosted),
+  // dinc = increment over U0 (interior), snext = next stage state (ghosted)
+  void* lo;
+  void* dinc;
+  void* snext;
+  int comp;
+};
+
+// ---------------------------------------------------------------------------
+// Closure (navier_stokes.py:44-156): P on interior + face ghosts, T interior.
+// ---------------------------------------------------------------------------
+// :100  P = (ga-1)*rho*(E - (v1**2+v2**2+v3**2)/2), from the cell's U.
+template <class R>
+MFVM_DEV R closure_p(const NSPar& p, R rho, R v1, R v2, R v3, R rhoE) {
+  const R E = rhoE / rho;
+  return (R(p.gm1) * rho) * (E - (((v1 * v1) + (v2 * v2)) + (v3 * v3)) / R(2));
+}
+
+// :94-97 primitive velocity of the cell at offset o.
+template <class R>
+MFVM_DEV void load_v(const R* __restrict__ U, long long o, long long fs, R& rho, R v[3]) {
+  rho = U[o];
+  v[0] = U[o + fs] / rho;
+  v[1] = U[o + 2 * fs] / rho;
+  v[2] = U[o + 3 * fs] / rho;
+}
+
+// :111-139 Stokes tensor from the centred differences dv[a][b] = v_a(+b) -
+// v_a(-b) and the cell's own velocity; writes the six unique entries.
+template <bool UNIF, class R>
+MFVM_DEV void closure_t(const Geo& geo, const NSPar& p, int i, int j, int k, const R dv[3][3],
+                        const R v[3], R* __restrict__ C, long long c, long long fs) {
+  R g[3][3];
+#pragma unroll
+  for (int b = 0; b < 3; ++b) {
+    if (geo.lpow2[b]) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) g[a][b] = dv[a][b] * R(0.5 * geo.linv[b]);
+    } else {
+      const R twoL = R(2) * g_len<UNIF, R>(geo, b, i, j, k);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) g[a][b] = dv[a][b] / twoL;
+    }
+  }
+  const R v1 = v[0], v2 = v[1], v3 = v[2];
+  const R mu = R(p.mu);
+  const R two3 = R(2.0 / 3.0);
+  R t11, t22, t33, t12, t13, t23;
+  if (UNIF || !geo.curv) {
+    // Cartesian: every Lame term is exactly +-0 and drops out of the sums.
+    const R divv = (g[0][0] + g[1][1]) + g[2][2];
+    t11 = mu * ((R(2) * g[0][0]) - (two3 * divv));
+    t22 = mu * ((R(2) * g[1][1]) - (two3 * divv));
+    t33 = mu * ((R(2) * g[2][2]) - (two3 * divv));
+    t12 = mu * (g[0][1] + g[1][0]);
+    t13 = mu * (g[0][2] + g[2][0]);
+    t23 = mu * (g[1][2] + g[2][1]);
+  } else {
+    const int a = i - 1, b = j - 1, e = k - 1;
+    const R h1 = met<R>(geo, MFVM_MET_H1, a, b, e), h2 = met<R>(geo, MFVM_MET_H2, a, b, e),
+            h3 = met<R>(geo, MFVM_MET_H3, a, b, e);
+    const R d1h2 = met<R>(geo, MFVM_MET_D1H2, a, b, e), d1h3 = met<R>(geo, MFVM_MET_D1H3, a, b, e);
+    const R d2h1 = met<R>(geo, MFVM_MET_D2H1, a, b, e), d2h3 = met<R>(geo, MFVM_MET_D2H3, a, b, e);
+    const R d3h1 = met<R>(geo, MFVM_MET_D3H1, a, b, e), d3h2 = met<R>(geo, MFVM_MET_D3H2, a, b, e);
+    // :128
+    const R divv = (((((g[0][0] + g[1][1]) + g[2][2]) + (v1 * (d1h2 + d1h3)) / h1) +
+                     (v2 * (d2h1 + d2h3)) / h2) +
+                    (v3 * (d3h1 + d3h2)) / h3);
+    // :131-136
+    t11 = mu * ((R(2) * g[0][0]) - (two3 * divv));
+    t22 = mu * ((R(2) * (g[1][1] + (v1 * d1h2) / h1)) - (two3 * divv));
+    t33 = mu * ((R(2) * ((g[2][2] + (v1 * d1h3) / h1) + (v2 * d2h3) / h2)) - (two3 * divv));
+    t12 = mu * (((g[0][1] + g[1][0]) - (v2 * d1h2) / h1) + (v1 * d2h1) / h2);
+    t13 = mu * (((g[0][2] + g[2][0]) - (v3 * d1h3) / h1) + (v1 * d3h1) / h3);
+    t23 = mu * (((g[1][2] + g[2][1]) - (v3 * d2h3) / h2) + (v2 * d3h2) / h3);
+  }
+  C[c + C_T11 * fs] = t11;
+  C[c + C_T22 * fs] = t22;
+  C[c + C_T33 * fs] = t33;
+  C[c + C_T12 * fs] = t12;
+  C[c + C_T13 * fs] = t13;
+  C[c + C_T23 * fs] = t23;
+}
+
+// Closure marching in x3.  One thread per (x1, x2) column of a BX x BY tile
+// (the workgroup shape) over a chunk of kz planes.  Each velocity is computed
+// once: the column keeps v at k-1, k, k+1 in registers, and plane k of the
+// tile plus a one-cell ring is exchanged through LDS for the x1 and x2
+// differences.  The ring cells are computed by helper threads; where a ring
+// cell is a face ghost of the domain, its helper also writes its P.  Without
+// viscosity only P is formed (on the same cells).  LDS: 3 (BY+2)(BX+2) R.
+template <bool VISC, bool UNIF, class R>
+MFVM_DEV void closure_march_body(const Layout& lay, const Geo& geo, const NSPar& p,
+                                 const R* __restrict__ U, R* __restrict__ C, int kz, TileId tl) {
+  extern __shared__ double smem_raw[];
+  R* vs = reinterpret_cast<R*>(smem_raw);  // [3][BY+2][BX+2]
+  const int BX = blockDim.x, BY = blockDim.y;
+  const int sx = BX + 2, splane = (BY + 2) * sx;
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * BX + tx;
+  const int n1 = lay.n1, n2 = lay.n2, n3 = lay.n3;
+  const long long fs = lay.fs;
+  const int i0 = 1 + tl.x * BX, j0 = 1 + tl.y * BY;
+  const int kb = 1 + tl.z * kz, ke = min(kb + kz, n3 + 1);
+  const int ilast = min(i0 + BX - 1, n1), jlast = min(j0 + BY - 1, n2);
+  const int i = i0 + tx, j = j0 + ty;
+  const bool own = i <= n1 && j <= n2;
+
+  // Ring cell of this helper thread (h < 2*BY + 2*BX), in global and LDS
+  // coordinates; hv = false when the ring slot lies beyond the tile's cells.
+  int hi = 0, hj = 0, hli = 0, hlj = 0;
+  bool hv = false;
+  {
+    const int h = tid;
+    if (h < BY) {
+      hj = j0 + h; hi = i0 - 1; hli = 0; hlj = h + 1; hv = hj <= jlast;
+    } else if (h < 2 * BY) {
+      hj = j0 + h - BY; hi = ilast + 1; hli = hi - i0 + 1; hlj = h - BY + 1; hv = hj <= jlast;
+    } else if (h < 2 * BY + BX) {
+      hi = i0 + h - 2 * BY; hj = j0 - 1; hli = hi - i0 + 1; hlj = 0; hv = hi <= ilast;
+    } else if (h < 2 * BY + 2 * BX) {
+      hi = i0 + h - 2 * BY - BX; hj = jlast + 1; hli = hi - i0 + 1; hlj = hj - j0 + 1; hv = hi <= ilast;
+    }
+  }
+  const bool hghost = hv && (hi == 0 || hi == n1 + 1 || hj == 0 || hj == n2 + 1);
+
+  R vm[3] = {R(0), R(0), R(0)}, vc[3] = {R(0), R(0), R(0)}, vn[3] = {R(0), R(0), R(0)};
+  if (own) {
+    R rho;
+    const long long om = lay.at(i, j, kb - 1), oc = lay.at(i, j, kb);
+    if (VISC || kb - 1 == 0) {
+      load_v(U, om, fs, rho, vm);
+      if (kb - 1 == 0) C[om] = closure_p(p, rho, vm[0], vm[1], vm[2], U[om + 4 * fs]);
+    }
+    load_v(U, oc, fs, rho, vc);
+    C[oc] = closure_p(p, rho, vc[0], vc[1], vc[2], U[oc + 4 * fs]);
+  }
+  for (int k = kb; k < ke; ++k) {
+    if (VISC) __syncthreads();  // the previous plane's LDS reads are done
+    if (VISC && own) {
+      const int l = (ty + 1) * sx + (tx + 1);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) vs[a * splane + l] = vc[a];
+    }
+    if (hv && (VISC || hghost)) {
+      const long long oh = lay.at(hi, hj, k);
+      R rho, vh[3];
+      load_v(U, oh, fs, rho, vh);
+      if (VISC) {
+        const int l = hlj * sx + hli;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) vs[a * splane + l] = vh[a];
+      }
+      if (hghost) C[oh] = closure_p(p, rho, vh[0], vh[1], vh[2], U[oh + 4 * fs]);
+    }
+    const bool pnext = k + 1 < ke || k + 1 == n3 + 1;
+    if (own && (VISC || pnext)) {
+      const long long on = lay.at(i, j, k + 1);
+      R rho;
+      load_v(U, on, fs, rho, vn);
+      if (pnext) C[on] = closure_p(p, rho, vn[0], vn[1], vn[2], U[on + 4 * fs]);
+    }
+    if constexpr (VISC) {
+      __syncthreads();
+      if (own) {
+        const int l = (ty + 1) * sx + (tx + 1);
+        R dv[3][3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          const R* va = vs + a * splane;
+          dv[a][0] = va[l + 1] - va[l - 1];
+          dv[a][1] = va[l + sx] - va[l - sx];
+          dv[a][2] = vn[a] - vm[a];
+        }
+        closure_t<UNIF, R>(geo, p, i, j, k, dv, vc, C, lay.at(i, j, k), fs);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          vm[a] = vc[a];
+          vc[a] = vn[a];
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Reconstruction and face flux
+// ---------------------------------------------------------------------------
+
+template <int D, class R>
+MFVM_DEV void load_q(const R* __restrict__ U, const R* __restrict__ C, long long fs, long long o,
+                     bool visc, R q[NQ]) {
+#pragma unroll
+  for (int f = 0; f < 5; ++f) q[f] = U[o + f * fs];
+  q[5] = C[o];
+  if (visc) {
+#pragma unroll
+    for (int e = 0; e < 3; ++e) q[6 + e] = C[o + trow(D, e) * fs];
+  } else {
+#pragma unroll
+    for (int e = 0; e < 3; ++e) q[6 + e] = R(0);
+  }
+}
+
+// Cell length along D at ghosted cell (i,j,k) (interior only).
+template <int D, bool UNIF, class R>
+MFVM_DEV R cell_len(const Geo& g, int i, int j, int k) {
+  if (g.lpow2[D]) return R(g.lval[D]);
+  return g_len<UNIF, R>(g, D, i, j, k);
+}
+
+// reconstructors.py:115-148 for one cell: limited slope from (qm, q, qp),
+// face states q + s/2 (left side of the cell's upper face) and q - s/2.
+template <int D, int LIM, bool UNIF, class R>
+MFVM_DEV void slope_states(const Geo& g, int i, int j, int k, int nf, const R qm[NQ],
+                           const R q[NQ], const R qp[NQ], R lo[NQ], R hi[NQ]) {
+  const R len = cell_len<D, UNIF, R>(g, i, j, k);
+#pragma unroll
+  for (int f = 0; f < NQ; ++f) {
+    if (f < nf) {
+      const R b = div_len<R>(g, D, q[f] - qm[f], len);
+      const R fw = div_len<R>(g, D, qp[f] - q[f], len);
+      const R s = limit<LIM, R>(b, fw) * len;
+      hi[f] = q[f] + s / R(2);
+      lo[f] = q[f] - s / R(2);
+    } else {
+      hi[f] = q[f];
+      lo[f] = q[f];
+    }
+  }
+}
+
+// States of ghosted cell (i,j,k) along D, loading the stencil from memory.
+// hi: state on the cell's upper face (reference fields_L), lo: on its lower
+// face (fields_R).  Ghost cells carry zero slope (reconstructors.py:127-132).
+template <int D, int LIM, bool UNIF, class R>
+MFVM_DEV void cell_states(const StageArgs& a, const R* U, const R* C, int i, int j, int k,
+                          R lo[NQ], R hi[NQ]) {
+  const long long fs = a.lay.fs;
+  const long long st = D == 0 ? 1 : (D == 1 ? a.lay.sy : a.lay.sz);
+  const long long o = a.lay.at(i, j, k);
+  R q[NQ];
+  load_q<D, R>(U, C, fs, o, a.p.visc, q);
+  const int cd = D == 0 ? i : (D == 1 ? j : k);
+  const int nd = D == 0 ? a.lay.n1 : (D == 1 ? a.lay.n2 : a.lay.n3);
+  if (LIM == MFVM_LIM_NONE || cd == 0 || cd == nd + 1) {
+#pragma unroll
+    for (int f = 0; f < NQ; ++f) lo[f] = hi[f] = q[f];
+    return;
+  }
+  R qm[NQ], qp[NQ];
+  load_q<D, R>(U, C, fs, o - st, a.p.visc, qm);
+  load_q<D, R>(U, C, fs, o + st, a.p.visc, qp);
+  slope_states<D, LIM, UNIF, R>(a.geo, i, j, k, a.p.visc ? 9 : 6, qm, q, qp, lo, hi);
+}
+
+// Normal flux F.S through a D-face from left state l and right state r:
+// convective flux (navier_stokes.py:195-212), Riemann solver
+// (riemann_solvers.py), centred viscous flux (navier_stokes.py:259-275,
+// diffusive_solvers.py:28), total F = Fc + Fd (:680-682), times the area.
+template <int D, int RS, class R>
+MFVM_DEV void face_flux(const NSPar& p, const R l[NQ], const R r[NQ], R area, R fsout[5]) {
+  const R rl = l[0], rr = r[0];
+  const R vl[3] = {l[1] / rl, l[2] / rl, l[3] / rl};
+  const R vr[3] = {r[1] / rr, r[2] / rr, r[3] / rr};
+  const R El = l[4] / rl, Er = r[4] / rr;
+  const R Pl = l[5], Pr = r[5];
+  R fl[5], fr[5];
+  fl[0] = rl * vl[D];
+  fr[0] = rr * vr[D];
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    fl[1 + e] = (rl * vl[e]) * vl[D];
+    fr[1 + e] = (rr * vr[e]) * vr[D];
+    if (e == D) {
+      fl[1 + e] = fl[1 + e] + Pl;
+      fr[1 + e] = fr[1 + e] + Pr;
+    }
+  }
+  fl[4] = ((rl * El) + Pl) * vl[D];
+  fr[4] = ((rr * Er) + Pr) * vr[D];
+  // :309  cs = sqrt(ga*P/rho)
+  const R csl = sqrt((R(p.gamma) * Pl) / rl);
+  const R csr = sqrt((R(p.gamma) * Pr) / rr);
+  R F[5];
+  if constexpr (RS == MFVM_RS_HLL) {
+    // Wave set {vn, vn-cs, vn+cs}: its min is vn-cs and its max vn+cs.
+    const R smin = nmin(vl[D] - csl, vr[D] - csr);
+    const R smax = nmax(vl[D] + csl, vr[D] + csr);
+    if (smin >= R(0)) {
+#pragma unroll
+      for (int q = 0; q < 5; ++q) F[q] = fl[q];
+    } else if (smax <= R(0)) {
+#pragma unroll
+      for (int q = 0; q < 5; ++q) F[q] = fr[q];
+    } else {
+      const R den = smax - smin;
+      const R sm = smin * smax;
+#pragma unroll
+      for (int q = 0; q < 5; ++q) F[q] = ((smax * fl[q] - smin * fr[q]) + sm * (r[q] - l[q])) / den;
+    }
+  } else {
+    // max |{vn, vn-cs, vn+cs}| is |vn| + cs, rounded once.
+    const R s = nmax(fabs(vl[D]) + csl, fabs(vr[D]) + csr);
+#pragma unroll
+    for (int q = 0; q < 5; ++q) F[q] = ((fl[q] + fr[q]) / R(2)) - ((s * (r[q] - l[q])) / R(2));
+  }
+  if (p.visc) {
+    // row D of T: l[6..8] = T_D1, T_D2, T_D3
+    const R el = ((-l[6]) * vl[0] - l[7] * vl[1]) - l[8] * vl[2];
+    const R er = ((-r[6]) * vr[0] - r[7] * vr[1]) - r[8] * vr[2];
+#pragma unroll
+    for (int e = 0; e < 3; ++e) F[1 + e] = F[1 + e] + ((-l[6 + e]) + (-r[6 + e])) / R(2);
+    F[4] = F[4] + (el + er) / R(2);
+  }
+#pragma unroll
+  for (int q = 0; q < 5; ++q) fsout[q] = F[q] * area;
+}
+
+// Source terms at interior cell (i,j,k) (navier_stokes.py:358-509), given
+// the 12 stored quantities of the cell: U(5), P, T11,T22,T33,T12,T13,T23.
+template <bool UNIF, class R>
+MFVM_DEV void cell_source(const Geo& g, const NSPar& p, int i, int j, int k, const R* u,
+                          const R* c, R src[5], bool& any) {
+  any = false;
+#pragma unroll
+  for (int q = 0; q < 5; ++q) src[q] = R(0);
+  if ((UNIF || !g.curv) && !p.grav) return;
+  any = true;
+  const R rho = u[0];
+  const R v1 = u[1] / rho, v2 = u[2] / rho, v3 = u[3] / rho;
+  R sg1 = R(0), sg2 = R(0), sg3 = R(0);
+  if (!UNIF && g.curv) {
+    const int a = i - 1, b = j - 1, e = k - 1;
+    const R h1 = met<R>(g, MFVM_MET_H1, a, b, e), h2 = met<R>(g, MFVM_MET_H2, a, b, e),
+            h3 = met<R>(g, MFVM_MET_H3, a, b, e);
+    const R d1h = met<R>(g, MFVM_MET_D1H, a, b, e), d1h2 = met<R>(g, MFVM_MET_D1H2, a, b, e),
+            d1h3 = met<R>(g, MFVM_MET_D1H3, a, b, e);
+    const R d2h = met<R>(g, MFVM_MET_D2H, a, b, e), d2h1 = met<R>(g, MFVM_MET_D2H1, a, b, e),
+            d2h3 = met<R>(g, MFVM_MET_D2H3, a, b, e);
+    const R d3h = met<R>(g, MFVM_MET_D3H, a, b, e), d3h1 = met<R>(g, MFVM_MET_D3H1, a, b, e),
+            d3h2 = met<R>(g, MFVM_MET_D3H2, a, b, e);
+    const R P = c[C_P];
+    const R T11 = c[C_T11], T22 = c[C_T22], T33 = c[C_T33];
+    const R T12 = c[C_T12], T13 = c[C_T13], T23 = c[C_T23];
+    // :439-441
+    const R c1 = ((((P * d1h) / h1 + (((rho * v2) * v2) * d1h2) / h1) + (((rho * v3) * v3) * d1h3) / h1) -
+                  (((rho * v1) * v2) * d2h1) / h2) -
+                 (((rho * v1) * v3) * d3h1) / h3;
+    const R c2 = ((((P * d2h) / h2 + (((rho * v1) * v1) * d2h1) / h2) + (((rho * v3) * v3) * d2h3) / h2) -
+                  (((rho * v2) * v1) * d1h2) / h1) -
+                 (((rho * v2) * v3) * d3h2) / h3;
+    const R c3 = ((((P * d3h) / h3 + (((rho * v1) * v1) * d3h1) / h3) + (((rho * v2) * v2) * d3h2) / h3) -
+                  (((rho * v3) * v1) * d1h3) / h1) -
+                 (((rho * v3) * v2) * d2h3) / h2;
+    // :446-448
+    const R e1 = (((T12 * d2h1) / h2 + (T13 * d3h1) / h3) - (T22 * d1h2) / h1) - (T33 * d1h3) / h1;
+    const R e2 = (((T12 * d1h2) / h1 + (T23 * d3h2) / h3) - (T11 * d2h1) / h2) - (T33 * d2h3) / h2;
+    const R e3 = (((T13 * d1h3) / h1 + (T23 * d2h3) / h2) - (T11 * d3h1) / h3) - (T22 * d3h2) / h3;
+    sg1 = c1 + e1;
+    sg2 = c2 + e2;
+    sg3 = c3 + e3;
+  }
+  // :500-504 gravity; src = sg + sp (:691)
+  const R g1 = R(p.g[0]), g2 = R(p.g[1]), g3 = R(p.g[2]);
+  src[0] = R(0);
+  src[1] = sg1 + rho * g1;
+  src[2] = sg2 + rho * g2;
+  src[3] = sg3 + rho * g3;
+  src[4] = R(0) + rho * (((g1 * v1) + (g2 * v2)) + (g3 * v3));
+}
+
+// CFL rates of one cell (navier_stokes.py:549-575): max_d (|v_d|+cs)/L_d and mu/rho.
+template <bool UNIF, class R>
+MFVM_DEV void cell_cfl(const Geo& g, const NSPar& p, int i, int j, int k, const R u[5], R& rate,
+                       R& nu) {
+  const R rho = u[0];
+  const R v1 = u[1] / rho, v2 = u[2] / rho, v3 = u[3] / rho;
+  const R E = u[4] / rho;
+  const R P = (R(p.gm1) * rho) * (E - (((v1 * v1) + (v2 * v2)) + (v3 * v3)) / R(2));
+  const R cs = sqrt((R(p.gamma) * P) / rho);
+  const R a1 = fabs(v1) + cs, a2 = fabs(v2) + cs, a3 = fabs(v3) + cs;
+  const R r1 = div_len<R>(g, 0, a1, cell_len<0, UNIF, R>(g, i, j, k));
+  const R r2 = div_len<R>(g, 1, a2, cell_len<1, UNIF, R>(g, i, j, k));
+  const R r3 = div_len<R>(g, 2, a3, cell_len<2, UNIF, R>(g, i, j, k));
+  rate = nmax(nmax(r1, r2), r3);
+  nu = R(p.mu) / rho;
+}
+
+// Block max of (rate, nu) -> partials[2*block], NaN-propagating.
+template <class R>
+MFVM_DEV void block_max2(R rate, R nu, double* slot, double* sred) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    rate = nmax(rate, __shfl_xor(rate, o, 64));
+    nu = nmax(nu, __shfl_xor(nu, o, 64));
+  }
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  const int nw = (blockDim.x * blockDim.y + 63) / 64;
+  __syncthreads();
+  if ((tid & 63) == 0) {
+    sred[2 * (tid >> 6)] = double(rate);
+    sred[2 * (tid >> 6) + 1] = double(nu);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double a = sred[0], b = sred[1];
+    for (int w = 1; w < nw; ++w) {
+      a = nmax(a, sred[2 * w]);
+      b = nmax(b, sred[2 * w + 1]);
+    }
+    slot[0] = a;
+    slot[1] = b;
+  }
+}
+
+// Linear id of this workgroup in its launch grid.
+MFVM_DEV long long hw_block_id() {
+  return (long long)(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+}
+
+// ---------------------------------------------------------------------------
+// Fused stage kernel
+// ---------------------------------------------------------------------------
+
+// The tile one workgroup advances: tile coordinates (x1, x2 tile and x3
+// chunk), tile shape (bx * by threads take part, bx fastest), and where the
+// tile's CFL maxima go (NULL: no CFL reduction).
+struct TileCtx {
+  int x, y, z;
+  int bx, by;
+  double* slot;
+};
+
+// Tile of a one-workgroup-per-tile launch over one grid.
+MFVM_DEV TileCtx grid_tile(const StageArgs& a) {
+  const TileId t = tile_id(a.remap != 0);
+  TileCtx c;
+  c.x = t.x;
+  c.y = t.y;
+  c.z = t.z;
+  c.bx = blockDim.x;
+  c.by = blockDim.y;
+  c.slot = a.partials ? a.partials + 2 * hw_block_id() : nullptr;
+  return c;
+}
+template <int LIM, int RS, bool UNIF, class R, bool COMP>
+MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl) {
+  extern __shared__ double smem_raw[];
+  R* sm = reinterpret_cast<R*>(smem_raw);
+  const R* __restrict__ U = static_cast<const R*>(a.U);
+  const R* __restrict__ C = static_cast<const R*>(a.C);
+  const int BX = tl.bx, BY = tl.by;
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  const int tx = tid % BX, ty = tid / BX;
+  const int n1 = a.lay.n1, n2 = a.lay.n2, n3 = a.lay.n3;
+  const long long fs = a.lay.fs;
+  const int i0 = 1 + tl.x * BX, j0 = 1 + tl.y * BY;
+  const int i = i0 + tx, j = j0 + ty;
+  const int kb = 1 + tl.z * a.kz;
+  const int ke = min(kb + a.kz, n3 + 1);
+  const bool own = (i <= n1) && (j <= n2);
+  const bool visc = a.p.visc != 0;
+
+  // LDS carve-up (elements of R): x1 and x2 exchange buffers side by side.
+  // RX[9][BY][BX+1] lower-face states, LX[9][BY] upper-face states of the
+  // x1 tile-edge cells, FX[5][BY][BX+1] face fluxes; RY/LY/FY likewise for x2.
+  const int wx = BX + 1;
+  const int nrx = BY * wx, nry = (BY + 1) * BX;
+  R* RX = sm;
+  R* LX = RX + NQ * nrx;
+  R* FX = LX + NQ * BY;
+  R* RY = FX + 5 * nrx;
+  R* LY = RY + NQ * nry;
+  R* FY = LY + NQ * BX;
+  const int NT = BX * BY;
+
+  double dt = a.dt_dev ? *a.dt_dev : a.dt;
+
+  // z-window of the own column: prev (ZS of m-1), cur (all of m), next (all of m+1)
+  R zp[NQ], cu[12], nx[12];
+  R lz_prev[NQ];            // upper-face state of cell m-1 (left state of face m-1/2)
+  R fsz_prev[5];            // F.S of face m-3/2
+  R acc_prev[5], src_prev[5];
+  bool src_any_prev = false;
+  double rate_max = 0.0, nu_max = 0.0;
+
+  auto load12 = [&](int k, R* w) {
+    const long long o = a.lay.at(i, j, k);
+#pragma unroll
+    for (int f = 0; f < 5; ++f) w[f] = U[o + f * fs];
+#pragma unroll
+    for (int f = 0; f < NS_NC; ++f) w[5 + f] = (f == 0 || visc) ? C[o + f * fs] : R(0);
+  };
+  // ZS of a 12-vector: U, P, T13, T23, T33
+  auto zs = [&](const R* w, R* z) {
+#pragma unroll
+    for (int f = 0; f < 6; ++f) z[f] = w[f];
+    z[6] = w[5 + C_T13];
+    z[7] = w[5 + C_T23];
+    z[8] = w[5 + C_T33];
+  };
+
+  if (own) {
+    R w0[12];
+    load12(kb - 1, w0);
+    zs(w0, zp);
+    load12(kb, cu);
+    if (kb - 1 == 0 || LIM == MFVM_LIM_NONE) {
+#pragma unroll
+      for (int f = 0; f < NQ; ++f) lz_prev[f] = zp[f];
+    } else {
+      R wm[12], zm[NQ], zc[NQ], lo[NQ];
+      load12(kb - 2, wm);
+      zs(wm, zm);
+      zs(cu, zc);
+      slope_states<2, LIM, UNIF, R>(a.geo, i, j, kb - 1, visc ? 9 : 6, zm, zp, zc, lo, lz_prev);
+    }
+  }
+
+  for (int m = kb; m <= ke; ++m) {
+    // ---- x3 face m-1/2 -------------------------------------------------------
+    R fsz[5];
+    R lz_cur[NQ];
+    if (own) {
+      R zc[NQ], lo[NQ];
+      zs(cu, zc);
+      if (m + 1 <= n3 + 1) load12(m + 1, nx);
+      if (m == n3 + 1 || LIM == MFVM_LIM_NONE) {
+#pragma unroll
+        for (int f = 0; f < NQ; ++f) lo[f] = lz_cur[f] = zc[f];
+      } else {
+        R zn[NQ];
+        zs(nx, zn);
+        slope_states<2, LIM, UNIF, R>(a.geo, i, j, m, visc ? 9 : 6, zp, zc, zn, lo, lz_cur);
+      }
+      const R s3 = g_area<UNIF, R>(a.geo, 2, i - 1, j - 1, m - 1);
+      face_flux<2, RS, R>(a.p, lz_prev, lo, s3, fsz);
+    }
+
+    // ---- in-plane faces of plane m (all threads take part in the barriers) ----
+    // Phase 1: x1 and x2 states of the own cells and of the tile-edge cells;
+    // phase 2: F.S of every x1 and x2 face of the tile, each computed once;
+    // phase 3: flux differences.  Helper tasks go to the first threads in
+    // phase 1 and to the last threads in phase 2, so that no wave takes two.
+    R accm[5], srcm[5];
+    bool src_any = false;
+    if (m < ke) {
+      R hx[NQ], hy[NQ];
+      if (j <= n2 && i <= n1 + 1) {
+        R lo[NQ];
+        if (i <= n1) {
+          // own cell: centre from the z-window, neighbours from memory
+          const long long o = a.lay.at(i, j, m);
+          R q[NQ];
+#pragma unroll
+          for (int f = 0; f < 6; ++f) q[f] = cu[f];
+          q[6] = cu[5 + C_T11];
+          q[7] = cu[5 + C_T12];
+          q[8] = cu[5 + C_T13];
+          if (LIM == MFVM_LIM_NONE) {
+#pragma unroll
+            for (int f = 0; f < NQ; ++f) lo[f] = hx[f] = q[f];
+          } else {
+            R qm[NQ], qp[NQ];
+            load_q<0, R>(U, C, fs, o - 1, visc, qm);
+            load_q<0, R>(U, C, fs, o + 1, visc, qp);
+            slope_states<0, LIM, UNIF, R>(a.geo, i, j, m, visc ? 9 : 6, qm, q, qp, lo, hx);
+          }
+        } else {
+          cell_states<0, LIM, UNIF, R>(a, U, C, i, j, m, lo, hx);  // x1 ghost: raw
+        }
+#pragma unroll
+        for (int f = 0; f < NQ; ++f) RX[f * nrx + ty * wx + tx] = lo[f];
+      }
+      if (i <= n1 && j <= n2 + 1) {
+        R lo[NQ];
+        if (j <= n2) {
+          const long long o = a.lay.at(i, j, m);
+          R q[NQ];
+#pragma unroll
+          for (int f = 0; f < 6; ++f) q[f] = cu[f];
+          q[6] = cu[5 + C_T12];
+          q[7] = cu[5 + C_T22];
+          q[8] = cu[5 + C_T23];
+          if (LIM == MFVM_LIM_NONE) {
+#pragma unroll
+            for (int f = 0; f < NQ; ++f) lo[f] = hy[f] = q[f];
+          } else {
+            R qm[NQ], qp[NQ];
+            load_q<1, R>(U, C, fs, o - a.lay.sy, visc, qm);
+            load_q<1, R>(U, C, fs, o + a.lay.sy, visc, qp);
+            slope_states<1, LIM, UNIF, R>(a.geo, i, j, m, visc ? 9 : 6, qm, q, qp, lo, hy);
+          }
+        } else {
+          cell_states<1, LIM, UNIF, R>(a, U, C, i, j, m, lo, hy);  // x2 ghost: raw
+        }
+#pragma unroll
+        for (int f = 0; f < NQ; ++f) RY[f * nry + ty * BX + tx] = lo[f];
+      }
+      // helpers, phase 1: [0, 2BX) x2 tile-edge cells, [2BX, 2BX+2BY) x1 tile-edge cells
+      for (int t = tid; t < 2 * (BX + BY); t += NT) {
+        R lo[NQ], hi[NQ];
+        if (t < 2 * BX) {
+          const int c = t % BX, side = t / BX, ii = i0 + c;
+          if (ii <= n1) {
+            if (side == 0) {
+              cell_states<1, LIM, UNIF, R>(a, U, C, ii, j0 - 1, m, lo, hi);
+#pragma unroll
+              for (int f = 0; f < NQ; ++f) LY[f * BX + c] = hi[f];
+            } else if (j0 + BY <= n2 + 1) {
+              cell_states<1, LIM, UNIF, R>(a, U, C, ii, j0 + BY, m, lo, hi);
+#pragma unroll
+              for (int f = 0; f < NQ; ++f) RY[f * nry + BY * BX + c] = lo[f];
+            }
+          }
+        } else {
+          const int u = t - 2 * BX, r = u % BY, side = u / BY, jj = j0 + r;
+          if (jj <= n2) {
+            if (side == 0) {
+              cell_states<0, LIM, UNIF, R>(a, U, C, i0 - 1, jj, m, lo, hi);
+#pragma unroll
+              for (int f = 0; f < NQ; ++f) LX[f * BY + r] = hi[f];
+            } else if (i0 + BX <= n1 + 1) {
+              cell_states<0, LIM, UNIF, R>(a, U, C, i0 + BX, jj, m, lo, hi);
+#pragma unroll
+              for (int f = 0; f < NQ; ++f) RX[f * nrx + r * wx + BX] = lo[f];
+            }
+          }
+        }
+      }
+      __syncthreads();
+      if (own) {
+        R rr[NQ], fo[5];
+#pragma unroll
+        for (int f = 0; f < NQ; ++f) rr[f] = RX[f * nrx + ty * wx + tx + 1];
+        face_flux<0, RS, R>(a.p, hx, rr, g_area<UNIF, R>(a.geo, 0, i, j - 1, m - 1), fo);
+#pragma unroll
+        for (int q = 0; q < 5; ++q) FX[q * nrx + ty * wx + tx + 1] = fo[q];
+#pragma unroll
+        for (int f = 0; f < NQ; ++f) rr[f] = RY[f * nry + (ty + 1) * BX + tx];
+        face_flux<1, RS, R>(a.p, hy, rr, g_area<UNIF, R>(a.geo, 1, i - 1, j, m - 1), fo);
+#pragma unroll
+        for (int q = 0; q < 5; ++q) FY[q * nry + (ty + 1) * BX + tx] = fo[q];
+      }
+      // helpers, phase 2 (last threads): [0, BX) faces j0-1/2, [BX, BX+BY) faces i0-1/2
+      for (int u = NT - 1 - tid; u < BX + BY; u += NT) {
+        R ll[NQ], rr[NQ], fo[5];
+        if (u < BX) {
+          const int c = u, ii = i0 + c;
+          if (ii <= n1) {
+#pragma unroll
+            for (int f = 0; f < NQ; ++f) {
+              ll[f] = LY[f * BX + c];
+              rr[f] = RY[f * nry + c];
+            }
+            face_flux<1, RS, R>(a.p, ll, rr, g_area<UNIF, R>(a.geo, 1, ii - 1, j0 - 1, m - 1), fo);
+#pragma unroll
+            for (int q = 0; q < 5; ++q) FY[q * nry + c] = fo[q];
+          }
+        } else {
+          const int r = u - BX, jj = j0 + r;
+          if (jj <= n2) {
+#pragma unroll
+            for (int f = 0; f < NQ; ++f) {
+              ll[f] = LX[f * BY + r];
+              rr[f] = RX[f * nrx + r * wx];
+            }
+            face_flux<0, RS, R>(a.p, ll, rr, g_area<UNIF, R>(a.geo, 0, i0 - 1, jj - 1, m - 1), fo);
+#pragma unroll
+            for (int q = 0; q < 5; ++q) FX[q * nrx + r * wx] = fo[q];
+          }
+        }
+      }
+      __syncthreads();
+      if (own) {
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+          const R ax = FX[q * nrx + ty * wx + tx + 1] - FX[q * nrx + ty * wx + tx];
+          accm[q] = (ax + FY[q * nry + (ty + 1) * BX + tx]) - FY[q * nry + ty * BX + tx];
+        }
+        cell_source<UNIF, R>(a.geo, a.p, i, j, m, cu, cu + 5, srcm, src_any);
+      }
+    }
+
+    // ---- finalize cell m-1 -----------------------------------------------------
+    if (own && m > kb) {
+      const int k = m - 1;
+      R kr[5];
+      R vol;
+      const bool vp2 = a.geo.vpow2 != 0;
+      if (!vp2) vol = g_vol<UNIF, R>(a.geo, i, j, k);
+#pragma unroll
+      for (int q = 0; q < 5; ++q) {
+        const R x = -((acc_prev[q] + fsz[q]) - fsz_prev[q]);
+        kr[q] = vp2 ? x * R(a.geo.vinv) : x / vol;
+        if (src_any_prev) kr[q] = kr[q] + src_prev[q];
+      }
+      const long long oc = a.lay.at(i, j, k);
+      const long long oi = a.il.at(i, j, k);
+      R* out = static_cast<R*>(a.out);
+      R* ka = static_cast<R*>(a.kacc);
+      const R* U0 = static_cast<const R*>(a.U0);
+      if constexpr (!COMP) {
+        R nu_[5];
+        const R dtr = R(dt);
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+          const R uq = zp[q];  // stage input U at cell m-1
+          R o = uq;
+          switch (a.update) {
+            case MFVM_UPD_EULER: o = uq + dtr * kr[q]; break;
+            case MFVM_UPD_BASE: o = U0[oc + q * fs] + R(a.c0 * dt) * kr[q]; break;
+            case MFVM_UPD_SSP: o = (R(a.c0) * U0[oc + q * fs]) + (R(a.c1) * (uq + dtr * kr[q])); break;
+            case MFVM_UPD_ACC6: o = U0[oc + q * fs] + R(dt / 6.0) * (ka[oi + q * a.il.fs] + kr[q]); break;
+            default: break;
+          }
+          nu_[q] = o;
+          if (a.update != MFVM_UPD_NONE) out[oc + q * fs] = o;
+          if (a.acc == MFVM_ACC_SET) ka[oi + q * a.il.fs] = kr[q];
+          else if (a.acc == MFVM_ACC_ADD) ka[oi + q * a.il.fs] = ka[oi + q * a.il.fs] + R(a.acc_coef) * kr[q];
+        }
+        if (a.partials) {
+          R rate, nu;
+          cell_cfl<UNIF, R>(a.geo, a.p, i, j, k, nu_, rate, nu);
+          rate_max = nmax(rate_max, double(rate));
+          nu_max = nmax(nu_max, double(nu));
+        }
+      } else {
+        // Compensated update: U0 = hi + lo (two FP32 words), stage states are
+        // U0 + d with an FP32 increment d; the last stage adds its increment
+        // to hi + lo in FP64 and splits the sum back into (hi, lo).
+        R* lo = static_cast<R*>(a.lo);
+        R* dv = static_cast<R*>(a.dinc);
+        R* sn = static_cast<R*>(a.snext);
+        const bool fin = a.comp >= MFVM_COMP_FINAL_SET;
+        double nw[5];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+          const R kq = kr[q];
+          const long long og = oc + q * fs, ii = oi + q * a.il.fs;
+          const R h = U0[og], l = lo[og];
+          if (!fin) {
+            R d;
+            if (a.comp == MFVM_COMP_SET) d = R(a.c0 * dt) * kq;
+            else d = R(a.c0) * (dv[ii] + R(dt) * kq);
+            dv[ii] = d;
+            sn[og] = h + (l + d);
+          } else {
+            double inc;
+            if (a.comp == MFVM_COMP_FINAL_SET) inc = (a.c0 * dt) * double(kq);
+            else if (a.comp == MFVM_COMP_FINAL_SSP) inc = a.c0 * (double(dv[ii]) + dt * double(kq));
+            else inc = (dt / 6.0) * (double(ka[ii]) + double(kq));
+            const double v = (double(h) + double(l)) + inc;
+            const R nh = R(v);
+            out[og] = nh;
+            lo[og] = R(v - double(nh));
+            nw[q] = v;
+          }
+          if (a.acc == MFVM_ACC_SET) ka[ii] = kq;
+          else if (a.acc == MFVM_ACC_ADD) ka[ii] = ka[ii] + R(a.acc_coef) * kq;
+        }
+        if (fin && a.partials) {
+          double rate, nu;
+          cell_cfl<UNIF, double>(a.geo, a.p, i, j, k, nw, rate, nu);
+          rate_max = nmax(rate_max, rate);
+          nu_max = nmax(nu_max, nu);
+        }
+      }
+    }
+
+    // ---- rotate the window -------------------------------------------------------
+    if (own) {
+#pragma unroll
+      for (int q = 0; q < 5; ++q) fsz_prev[q] = fsz[q];
+#pragma unroll
+      for (int f = 0; f < NQ; ++f) lz_prev[f] = lz_cur[f];
+      if (m < ke) {
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+          acc_prev[q] = accm[q];
+          src_prev[q] = srcm[q];
+        }
+        src_any_prev = src_any;
+      }
+      zs(cu, zp);
+#pragma unroll
+      for (int f = 0; f < 12; ++f) cu[f] = nx[f];
+    }
+  }
+
+  if (tl.slot) block_max2<double>(rate_max, nu_max, tl.slot, reinterpret_cast<double*>(smem_raw));
+}
+
+// CFL maxima of one tile (navier_stokes.py:512-583): rate_c and nu over the
+// tile's cells, reduced to the tile's slot.  lo != NULL: compensated FP32
+// state, rates in FP64 of hi + lo.
+template <bool UNIF, class R>
+MFVM_DEV void cfl_tile_body(const Layout& lay, const Geo& geo, const NSPar& p, const R* U, const R* lo,
+                            int kz, const TileCtx& tl) {
+  extern __shared__ double smem_raw[];
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  const int i = 1 + tl.x * tl.bx + tid % tl.bx;
+  const int j = 1 + tl.y * tl.by + tid / tl.bx;
+  const int kb = 1 + tl.z * kz, ke = min(kb + kz, lay.n3 + 1);
+  double rate_max = 0.0, nu_max = 0.0;
+  if (i <= lay.n1 && j <= lay.n2) {
+    for (int k = kb; k < ke; ++k) {
+      const long long o = lay.at(i, j, k);
+      if (lo) {
+        double u[5];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) u[q] = double(U[o + q * lay.fs]) + double(lo[o + q * lay.fs]);
+        double rate, nu;
+        cell_cfl<UNIF, double>(geo, p, i, j, k, u, rate, nu);
+        rate_max = nmax(rate_max, rate);
+        nu_max = nmax(nu_max, nu);
+      } else {
+        R u[5];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) u[q] = U[o + q * lay.fs];
+        R rate, nu;
+        cell_cfl<UNIF, R>(geo, p, i, j, k, u, rate, nu);
+        rate_max = nmax(rate_max, double(rate));
+        nu_max = nmax(nu_max, double(nu));
+      }
+    }
+  }
+  block_max2<double>(rate_max, nu_max, tl.slot, smem_raw);
+}
+
+// dt = cfl / (max rate_c + (2*max nu) * smax) over nblk partial pairs
+// (navier_stokes.py:571-583), reduced by one workgroup of <= 1024 threads.
+MFVM_DEV double dt_reduce_block(const double* partials, long long nblk, double cfl, double smax) {
+  __shared__ double sa[1024], sb[1024];
+  double a = 0.0, b = 0.0;
+  for (long long t = threadIdx.x; t < nblk; t += blockDim.x) {
+    a = nmax(a, partials[2 * t]);
+    b = nmax(b, partials[2 * t + 1]);
+  }
+  sa[threadIdx.x] = a;
+  sb[threadIdx.x] = b;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      sa[threadIdx.x] = nmax(sa[threadIdx.x], sa[threadIdx.x + s]);
+      sb[threadIdx.x] = nmax(sb[threadIdx.x], sb[threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  return cfl / (sa[0] + (2.0 * sb[0]) * smax);
+}
+
+// ---------------------------------------------------------------------------
+// Host-side helpers shared by the launchers
+// ---------------------------------------------------------------------------
+
+struct Tile {
+  int bx, by, kz;
+};
+
+inline Tile ns_tile(const mfvm_grid* g, const mfvm_ns_params* p) {
+  Tile t;
+  t.bx = p->tile[0] > 0 ? p->tile[0] : 64;
+  t.by = p->tile[1] > 0 ? p->tile[1] : 4;
+  t.kz = p->tile[2] > 0 ? p->tile[2] : (g->n[2] < 32 ? g->n[2] : 32);
+  return t;
+}
+
+// LDS bytes of the stage body for a bx x by tile.
+inline size_t stage_lds_bytes(int bx, int by, size_t rsize) {
+  const size_t nrx = (size_t)by * (bx + 1), nry = (size_t)(by + 1) * bx;
+  size_t lds = (NQ * nrx + NQ * by + 5 * nrx + NQ * nry + NQ * bx + 5 * nry) * rsize;
+  const size_t red = 2 * sizeof(double) * ((bx * by + 63) / 64);
+  return lds < red ? red : lds;
+}
+
+inline NSPar make_ns_par(const mfvm_ns_params* p) {
+  NSPar q;
+  q.gamma = p->gamma;
+  q.gm1 = p->gamma - 1.0;
+  q.mu = p->mu;
+  q.g[0] = p->g[0];
+  q.g[1] = p->g[1];
+  q.g[2] = p->g[2];
+  q.visc = p->mu != 0.0;
+  q.grav = (p->g[0] != 0.0) || (p->g[1] != 0.0) || (p->g[2] != 0.0);
+  return q;
+}
+
+// XCD-aware tile order (tile_id); MFVM_XCD_REMAP=0 selects the plain order.
+inline int xcd_remap_enabled() {
+  static const int on = [] {
+    const char* e = getenv("MFVM_XCD_REMAP");
+    return e ? atoi(e) : 1;
+  }();
+  return on;
+}
+
+}  // namespace mfvm

@@ -23,7 +23,7 @@ INT_EULER, INT_RK3, INT_SSPRK3, INT_RK4 = 0, 1, 2, 3
 F64, F32 = 0, 1
 UPD_NONE, UPD_EULER, UPD_BASE, UPD_SSP, UPD_ACC6 = range(5)
 ACC_NONE, ACC_SET, ACC_ADD = range(3)
-E_ARG, E_HIP, E_UNSUP = -1, -2, -3
+OK, E_ARG, E_HIP, E_UNSUP = 0, -1, -2, -3
 
 
 class Grid(ctypes.Structure):
@@ -78,6 +78,21 @@ class NSWork(ctypes.Structure):
                 ("partials", ctypes.c_void_p), ("dts", ctypes.c_void_p)]
 
 
+ENS_ORDER_MEMBER, ENS_ORDER_COST, ENS_ORDER_INTERLEAVE = 0, 1, 2
+
+
+class EnsMember(ctypes.Structure):
+    _fields_ = [("grid", Grid), ("bc", (ctypes.c_int32 * 6) * 5), ("tile", ctypes.c_int32 * 3),
+                ("U", ctypes.c_void_p), ("S1", ctypes.c_void_p), ("S2", ctypes.c_void_p),
+                ("C", ctypes.c_void_p), ("kacc", ctypes.c_void_p), ("lo", ctypes.c_void_p),
+                ("dinc", ctypes.c_void_p), ("partials", ctypes.c_void_p), ("dts", ctypes.c_void_p)]
+
+
+class EnsOpts(ctypes.Structure):
+    _fields_ = [("order", ctypes.c_int32), ("persist", ctypes.c_int32), ("graph", ctypes.c_int32),
+                ("remap", ctypes.c_int32)]
+
+
 # (name, restype, argtypes) of every exported entry point.
 _P = ctypes.c_void_p
 _G = ctypes.POINTER(Grid)
@@ -99,6 +114,12 @@ SIGNATURES = [
     ("mfvm_ns_partials_size", ctypes.c_int64, [_G, _NP]),
     ("mfvm_ns_comp_advance", ctypes.c_int, [_G, _NP, ctypes.POINTER(NSCompWork), ctypes.c_int32,
                                             ctypes.c_double, ctypes.c_int32, _P]),
+    ("mfvm_ens_member_tiles", ctypes.c_int64, [ctypes.POINTER(EnsMember)]),
+    ("mfvm_ens_table_bytes", ctypes.c_int64, [ctypes.POINTER(EnsMember), ctypes.c_int32,
+                                              ctypes.c_int32]),
+    ("mfvm_ens_advance", ctypes.c_int, [ctypes.POINTER(EnsMember), ctypes.c_int32, _NP,
+                                        ctypes.POINTER(EnsOpts), ctypes.c_int32, ctypes.c_int32,
+                                        ctypes.c_double, ctypes.c_int32, _P, ctypes.c_int64, _P]),
     ("mfvm_heat_rhs", ctypes.c_int, [_G, ctypes.POINTER(HeatParams), _P, _P, _P]),
     ("mfvm_heat_dt", ctypes.c_int, [_G, ctypes.POINTER(HeatParams), ctypes.c_double, _P, _P]),
     ("mfvm_heat_step", ctypes.c_int, [_G, ctypes.POINTER(HeatParams), ctypes.POINTER(NSWork),

@@ -106,3 +106,51 @@ def test_uniform_power_of_two_detection():
     cfg["x1_max"] = 0.7
     d = GridDesc("cartesian", cfg, "cpu").desc
     assert d.lpow2[0] == 0 and d.vpow2 == 0
+
+
+def test_ensemble_argument_checks():
+    """mfvm_ens_*: sizes, tiles, integrators and buffers are validated on the
+    host (no HIP call before the checks pass; nsteps=0 returns early)."""
+    from mfvm import _lib
+    from mfvm.grid.geometry import GridDesc
+    cfg = {"x1_min": 0.0, "x1_max": 1.0, "nx1": 100, "x2_min": 0.0, "x2_max": 1.0, "nx2": 30,
+           "x3_min": 0.0, "x3_max": 1.0, "nx3": 1}
+    gd = GridDesc("cartesian", cfg, "cpu")
+    m = (_lib.EnsMember * 2)()
+    for i in range(2):
+        m[i].grid = gd.desc
+        for v in range(5):
+            m[i].bc[v][:] = [1] * 6
+    assert _lib.lib.mfvm_ens_member_tiles(m) == 2 * 8          # 64 x 4 tiles
+    m[0].tile[:] = [32, 8, 0]
+    assert _lib.lib.mfvm_ens_member_tiles(m) == 4 * 4
+    m[0].tile[:] = [32, 4, 0]                                     # 128 threads: rejected
+    assert _lib.lib.mfvm_ens_member_tiles(m) == -1
+    assert "256" in _lib.lib.mfvm_last_error().decode()
+    m[0].tile[:] = [0, 0, 0]
+    assert _lib.lib.mfvm_ens_table_bytes(m, 2, _lib.INT_EULER) == -1
+    nb = _lib.lib.mfvm_ens_table_bytes(m, 2, _lib.INT_RK3)
+    assert nb > 0 and nb % 256 == 0
+    p = _lib.NSParams()
+    p.dtype = _lib.F64
+    p.limiter = _lib.LIM_MC
+    p.riemann = _lib.RS_RUSANOV
+    p.gamma = 1.4
+    o = _lib.EnsOpts()
+    args = lambda integ, comp, table, nbytes, steps: (m, 2, ctypes.byref(p), ctypes.byref(o), integ, comp,
+                                                      0.4, steps, table, nbytes, None)
+    # NULL member buffers
+    assert _lib.lib.mfvm_ens_advance(*args(_lib.INT_RK3, 0, 1, nb, 0)) == _lib.E_ARG
+    for i in range(2):
+        for f in ("U", "S1", "S2", "C", "kacc", "partials", "dts"):
+            setattr(m[i], f, 4096)
+    assert _lib.lib.mfvm_ens_advance(*args(_lib.INT_RK3, 0, 1, nb, 0)) == _lib.OK
+    assert _lib.lib.mfvm_ens_advance(*args(_lib.INT_EULER, 0, 1, nb, 0)) == _lib.E_UNSUP
+    assert _lib.lib.mfvm_ens_advance(*args(_lib.INT_RK3, 0, 1, nb - 1, 0)) == _lib.E_ARG
+    assert "table" in _lib.lib.mfvm_last_error().decode()
+    assert _lib.lib.mfvm_ens_advance(*args(_lib.INT_RK3, 1, 1, nb, 0)) == _lib.E_ARG   # comp needs f32
+    p.dtype = _lib.F32
+    assert _lib.lib.mfvm_ens_advance(*args(_lib.INT_RK3, 1, 1, nb, 0)) == _lib.E_ARG   # lo / dinc
+    o.order = 5
+    p.dtype = _lib.F64
+    assert _lib.lib.mfvm_ens_advance(*args(_lib.INT_RK3, 0, 1, nb, 0)) == _lib.E_ARG
