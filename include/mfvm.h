@@ -328,6 +328,85 @@ int mfvm_ens_advance(const mfvm_ens_member* m, int32_t n, const mfvm_ns_params* 
                      double cfl, int32_t nsteps, void* table, int64_t table_bytes,
                      void* stream);
 
+/* ---- learned flux-correction closure -----------------------------------------
+ * No counterpart in the reference.  A 3-layer 3x3 CNN (4 -> 32 -> 32 -> 8
+ * channels, ReLU after layers 1 and 2, periodic padding) maps the normalised
+ * primitives (rho, v1, v2, p) of a 2D periodic Cartesian state to
+ * corrections dF of the x1-face (channels 0-3) and x2-face (4-7) fluxes of
+ * mass, momentum 1, momentum 2 and energy at the faces i-1/2 and j-1/2 of
+ * each cell; the corrected right-hand side is k - div(dF), with
+ * dF = y * out_scale per field.  Tensors are channels-last
+ * [batch][nx2][nx1][C]; activations BF16, accumulation FP32 on MFMA; nx1 must
+ * be a multiple of 16.  Weights: FP32 masters [cout][cin][3][3]; the
+ * convolutions read the BF16 layouts written by mfvm_cnn_pack. */
+
+#define MFVM_CNN_FWD_RELU 0    /* y = relu(conv(x) + b), BF16 */
+#define MFVM_CNN_FWD_LINEAR 1  /* y = conv(x) + b, FP32 */
+#define MFVM_CNN_BWD_DATA 2    /* y = conv_transpose(x) * (mask > 0), BF16 */
+
+/* One convolution; (cin, cout) in {(4,32), (32,32), (32,8)} forward and
+ * {(32,32), (8,32)} for BWD_DATA (the transposed layers: x has cin = the
+ * layer's cout channels, w the backward layout, mask the layer input). */
+int mfvm_cnn_conv(int32_t mode, int32_t cin, int32_t cout, int32_t batch, int32_t n1, int32_t n2,
+                  const void* x, const void* w, const float* bias, const void* mask, void* y,
+                  void* stream);
+/* Accumulates weight / bias gradients of one layer into gw, gb (FP32). */
+int mfvm_cnn_wgrad(int32_t cin, int32_t cout, int32_t batch, int32_t n1, int32_t n2, const void* x,
+                   const void* gy, float* gw, float* gb, void* stream);
+/* FP32 master weights -> BF16 forward layout wf [9][round16(cout)][cin] and
+ * backward layout wb [9][round16(cin)][cout]. */
+int mfvm_cnn_pack(int32_t cin, int32_t cout, const float* w, void* wf, void* wb, void* stream);
+/* One Adam step (step number t >= 1) on n FP32 parameters. */
+int mfvm_cnn_adam(int64_t n, float* w, const float* g, float* m, float* v, float lr, float b1,
+                  float b2, float eps, int32_t t, void* stream);
+/* Network input from `batch` ghosted 2D states spaced batch_stride elements:
+ * x = ((rho, v1, v2, p) - shift) * scale, BF16 channels-last. */
+int mfvm_cnn_features(const mfvm_grid* g, int32_t dtype, double gamma, int32_t batch,
+                      int64_t batch_stride, const void* U, const float* shift, const float* scale,
+                      void* x, void* stream);
+/* Training loss sum_f wf[f] (kc_f - target_f)^2 averaged over the cells,
+ * added to *loss; gy = dloss/dy (BF16).  target: [batch][nx2][nx1][4] FP32. */
+int mfvm_cnn_loss(int32_t batch, int32_t n1, int32_t n2, double dx1, double dx2, const float* y,
+                  const float* target, const float* out_scale, const float* wf, void* gy,
+                  float* loss, void* stream);
+
+typedef struct mfvm_cnn {
+  const void* wf[3];            /* BF16 forward weights (mfvm_cnn_pack) */
+  const float* b[3];            /* FP32 biases: 32, 32, 8 */
+  const float* in_shift;        /* 4 floats (device) */
+  const float* in_scale;        /* 4 floats (device) */
+  const float* out_scale;       /* 4 floats (device) */
+} mfvm_cnn;
+
+typedef struct mfvm_closure_work {
+  void* x;                      /* BF16 nx1*nx2*4 */
+  void* a1;                     /* BF16 nx1*nx2*32 */
+  void* a2;                     /* BF16 nx1*nx2*32 */
+  float* y;                     /* FP32 nx1*nx2*8 */
+  void* T;                      /* ghosted 5-field state scratch */
+  int* flag;                    /* 3 ints: fallback flag of each stage of the current step */
+  int* counts;                  /* 3 ints: stages (1st, 2nd, 3rd) that fell back, accumulated */
+  double* dt_cur;               /* 1 double */
+  double* t_cur;                /* 1 double: time reached (from 0 at the first call's start) */
+  int* step;                    /* 1 int: dts written so far */
+} mfvm_closure_work;
+
+/* nsteps rk3_ssp steps (time_integrators.py:143-159 with
+ * navier_stokes.compute_rhs) whose every stage adds the network's correction
+ * -div(dF) to the right-hand side; a stage whose corrected state has rho <= 0
+ * or p <= 0 in any cell keeps the plain result instead (counted in
+ * cw->counts).  net == NULL: the plain scheme.  dt from the CFL rule on every
+ * new state, clipped so that the time never passes t_end (pass INFINITY for
+ * the reference's unclipped loop, src/solver.py:137-151); steps after t_end
+ * have dt = 0 and leave the state unchanged.  w->dts gets nsteps+1 entries.
+ * 2D (nx3 == 1), uniform Cartesian, periodic in x1 and x2, float64.
+ * graph > 0: steps per captured hipGraph (stream must not be the null
+ * stream; synchronises it before returning).  With an all-zero network and
+ * t_end = INFINITY the result equals mfvm_ns_advance bit for bit. */
+int mfvm_closure_advance(const mfvm_grid* g, const mfvm_ns_params* p, const mfvm_ns_work* w,
+                         const mfvm_cnn* net, const mfvm_closure_work* cw, double cfl,
+                         double t_end, int32_t nsteps, int32_t graph, void* stream);
+
 /* ---- heat equation (src/physics/heat.py) -------------------------------------- */
 
 /* Right-hand side into an interior buffer.  Replaces heat.py:354-473. */

@@ -19,6 +19,28 @@ hipError_t heat_launch_dt(const mfvm_grid*, const mfvm_heat_params*, double, dou
 template <class R> hipError_t maxwell_launch_res(const mfvm_grid*, const mfvm_maxwell_params*, const void*, void*, hipStream_t);
 template <class R> hipError_t maxwell_launch_stage(const mfvm_grid*, const mfvm_maxwell_params*, const void*, const void*, const StageIO&, hipStream_t);
 hipError_t maxwell_launch_dt(const mfvm_grid*, const mfvm_maxwell_params*, double, double*, hipStream_t);
+template <class R> hipError_t launch_cfl_partials(const mfvm_grid*, const mfvm_ns_params*, const void*, const void*, double*, hipStream_t);
+hipError_t launch_dt_append(const mfvm_grid*, const mfvm_ns_params*, const double*, double, double, double*, double*,
+                            double*, int*, hipStream_t);
+// Defined in closure.hip.
+hipError_t cnn_conv(int mode, int cin, int cout, int batch, int n1, int n2, const void* x, const void* w,
+                    const float* b, const void* mask, void* y, hipStream_t s);
+bool cnn_shape_ok(int mode, int cin, int cout);
+hipError_t cnn_wgrad(int cin, int cout, int batch, int n1, int n2, const void* x, const void* gy, float* gw,
+                     float* gb, hipStream_t s);
+hipError_t cnn_pack(int cin, int cout, const float* w, void* wf, void* wb, hipStream_t s);
+hipError_t cnn_adam(long long n, float* w, const float* g, float* m, float* v, float lr, float b1, float b2,
+                    float eps, int t, hipStream_t s);
+template <class R> hipError_t cnn_features(const mfvm_grid* g, double gamma, int batch, long long bs, const void* U,
+                                           const float* shift, const float* scale, void* x, hipStream_t s);
+hipError_t cnn_loss(int batch, int n1, int n2, float inv1, float inv2, const float* y, const float* target,
+                    const float* osc, const float* wf, void* gy, float* loss, hipStream_t s);
+template <class R> hipError_t closure_apply(const mfvm_grid* g, double gamma, const void* out, const float* y,
+                                            const float* osc, double coef, const double* dt, void* T, int* flag,
+                                            hipStream_t s);
+template <class R> hipError_t closure_select(const mfvm_grid* g, const void* T, void* out, const int* flag,
+                                             int* count, hipStream_t s);
+hipError_t fill_int(int* p, long long n, int v, hipStream_t s);
 // Defined in ensemble.hip.
 int ens_nstages(int integrator);
 long long ens_member_tiles(const mfvm_ens_member* m);
@@ -548,6 +570,175 @@ int mfvm_ens_advance(const mfvm_ens_member* m, int32_t n, const mfvm_ns_params* 
   return hip_check(ens_advance(m, n, p, o, integrator, compensated != 0, cfl, nsteps, table,
                                static_cast<hipStream_t>(stream)),
                    "ens_advance");
+}
+
+// ---- learned flux-correction closure ----
+static int check_cnn_dims(int32_t batch, int32_t n1, int32_t n2) {
+  if (batch < 1 || n1 < 16 || n2 < 1 || n1 % 16 != 0)
+    return fail(MFVM_E_ARG, "cnn: need batch >= 1, nx2 >= 1 and nx1 a positive multiple of 16");
+  return MFVM_OK;
+}
+
+int mfvm_cnn_conv(int32_t mode, int32_t cin, int32_t cout, int32_t batch, int32_t n1, int32_t n2, const void* x,
+                  const void* w, const float* bias, const void* mask, void* y, void* stream) {
+  int rc = check_cnn_dims(batch, n1, n2);
+  if (rc) return rc;
+  if (mode < MFVM_CNN_FWD_RELU || mode > MFVM_CNN_BWD_DATA) return fail(MFVM_E_ARG, "cnn_conv: bad mode");
+  if (!cnn_shape_ok(mode, cin, cout)) return fail(MFVM_E_UNSUP, "cnn_conv: unsupported channel counts");
+  if (!x || !w || !y || (mode != MFVM_CNN_BWD_DATA && !bias) || (mode == MFVM_CNN_BWD_DATA && !mask))
+    return fail(MFVM_E_ARG, "cnn_conv: NULL buffer");
+  return hip_check(cnn_conv(mode, cin, cout, batch, n1, n2, x, w, bias, mask, y, static_cast<hipStream_t>(stream)),
+                   "cnn_conv");
+}
+
+int mfvm_cnn_wgrad(int32_t cin, int32_t cout, int32_t batch, int32_t n1, int32_t n2, const void* x, const void* gy,
+                   float* gw, float* gb, void* stream) {
+  int rc = check_cnn_dims(batch, n1, n2);
+  if (rc) return rc;
+  if (!cnn_shape_ok(MFVM_CNN_FWD_RELU, cin, cout)) return fail(MFVM_E_UNSUP, "cnn_wgrad: unsupported channel counts");
+  if (!x || !gy || !gw || !gb) return fail(MFVM_E_ARG, "cnn_wgrad: NULL buffer");
+  return hip_check(cnn_wgrad(cin, cout, batch, n1, n2, x, gy, gw, gb, static_cast<hipStream_t>(stream)), "cnn_wgrad");
+}
+
+int mfvm_cnn_pack(int32_t cin, int32_t cout, const float* w, void* wf, void* wb, void* stream) {
+  if (!cnn_shape_ok(MFVM_CNN_FWD_RELU, cin, cout)) return fail(MFVM_E_UNSUP, "cnn_pack: unsupported channel counts");
+  if (!w || !wf || !wb) return fail(MFVM_E_ARG, "cnn_pack: NULL buffer");
+  return hip_check(cnn_pack(cin, cout, w, wf, wb, static_cast<hipStream_t>(stream)), "cnn_pack");
+}
+
+int mfvm_cnn_adam(int64_t n, float* w, const float* g, float* m, float* v, float lr, float b1, float b2, float eps,
+                  int32_t t, void* stream) {
+  if (n < 1 || !w || !g || !m || !v || t < 1) return fail(MFVM_E_ARG, "cnn_adam: bad arguments");
+  return hip_check(cnn_adam(n, w, g, m, v, lr, b1, b2, eps, t, static_cast<hipStream_t>(stream)), "cnn_adam");
+}
+
+int mfvm_cnn_features(const mfvm_grid* g, int32_t dtype, double gamma, int32_t batch, int64_t batch_stride,
+                      const void* U, const float* shift, const float* scale, void* x, void* stream) {
+  int rc = check_grid(g);
+  if (rc) return rc;
+  if (g->n[2] != 1) return fail(MFVM_E_UNSUP, "cnn_features: 2D grids only (nx3 == 1)");
+  if (batch < 1 || !U || !shift || !scale || !x) return fail(MFVM_E_ARG, "cnn_features: bad arguments");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == MFVM_F64)
+    return hip_check(cnn_features<double>(g, gamma, batch, batch_stride, U, shift, scale, x, s), "cnn_features");
+  if (dtype == MFVM_F32)
+    return hip_check(cnn_features<float>(g, gamma, batch, batch_stride, U, shift, scale, x, s), "cnn_features");
+  return fail(MFVM_E_ARG, "dtype must be 0 (f64) or 1 (f32)");
+}
+
+int mfvm_cnn_loss(int32_t batch, int32_t n1, int32_t n2, double dx1, double dx2, const float* y, const float* target,
+                  const float* out_scale, const float* wf, void* gy, float* loss, void* stream) {
+  if (batch < 1 || n1 < 1 || n2 < 1 || !(dx1 > 0) || !(dx2 > 0)) return fail(MFVM_E_ARG, "cnn_loss: bad sizes");
+  if (!y || !target || !out_scale || !wf || !gy || !loss) return fail(MFVM_E_ARG, "cnn_loss: NULL buffer");
+  return hip_check(cnn_loss(batch, n1, n2, (float)(1.0 / dx1), (float)(1.0 / dx2), y, target, out_scale, wf, gy, loss,
+                            static_cast<hipStream_t>(stream)),
+                   "cnn_loss");
+}
+
+// One rk3_ssp step (time_integrators.py:143-159) with the learned correction
+// in every stage, then the dt of the next step appended to w->dts.
+static int closure_step(const mfvm_grid* g, const mfvm_ns_params* p, const mfvm_ns_work* w, const mfvm_cnn* net,
+                        const mfvm_closure_work* cw, double cfl, double t_end, hipStream_t s) {
+  struct Q {
+    void* in;
+    void* out;
+    int update;
+    double c0, c1, coef;
+  };
+  const Q seq[3] = {{w->U, w->S1, MFVM_UPD_EULER, 0.0, 0.0, 1.0},
+                    {w->S1, w->S2, MFVM_UPD_SSP, 0.75, 0.25, 0.25},
+                    {w->S2, w->U, MFVM_UPD_SSP, 1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
+  const int n1 = g->n[0], n2 = g->n[1];
+  const int32_t* bc = &p->bc[0][0];
+  hipError_t e = fill_int(cw->flag, 3, 0, s);
+  for (int q = 0; q < 3 && e == hipSuccess; ++q) {
+    e = launch_bcs<double>(g, 5, bc, seq[q].in, seq[q].in, s);
+    if (e == hipSuccess) e = launch_closure<double>(g, p, seq[q].in, w->C, s);
+    mfvm_stage st = {};
+    st.U = seq[q].in;
+    st.C = w->C;
+    st.U0 = w->U;
+    st.out = seq[q].out;
+    st.kacc = w->kacc;
+    st.update = seq[q].update;
+    st.acc = MFVM_ACC_NONE;
+    st.c0 = seq[q].c0;
+    st.c1 = seq[q].c1;
+    st.dt_dev = cw->dt_cur;
+    if (e == hipSuccess) e = launch_stage<double>(g, p, &st, s);
+    if (!net) continue;
+    if (e == hipSuccess)
+      e = cnn_features<double>(g, p->gamma, 1, 0, seq[q].in, net->in_shift, net->in_scale, cw->x, s);
+    if (e == hipSuccess) e = cnn_conv(MFVM_CNN_FWD_RELU, 4, 32, 1, n1, n2, cw->x, net->wf[0], net->b[0], nullptr, cw->a1, s);
+    if (e == hipSuccess) e = cnn_conv(MFVM_CNN_FWD_RELU, 32, 32, 1, n1, n2, cw->a1, net->wf[1], net->b[1], nullptr, cw->a2, s);
+    if (e == hipSuccess) e = cnn_conv(MFVM_CNN_FWD_LINEAR, 32, 8, 1, n1, n2, cw->a2, net->wf[2], net->b[2], nullptr, cw->y, s);
+    if (e == hipSuccess)
+      e = closure_apply<double>(g, p->gamma, seq[q].out, cw->y, net->out_scale, seq[q].coef, cw->dt_cur, cw->T,
+                                cw->flag + q, s);
+    if (e == hipSuccess) e = closure_select<double>(g, cw->T, seq[q].out, cw->flag + q, cw->counts + q, s);
+  }
+  if (e == hipSuccess) e = launch_cfl_partials<double>(g, p, w->U, nullptr, w->partials, s);
+  if (e == hipSuccess) e = launch_dt_append(g, p, w->partials, cfl, t_end, w->dts, cw->dt_cur, cw->t_cur, cw->step, s);
+  return hip_check(e, "closure_advance");
+}
+
+int mfvm_closure_advance(const mfvm_grid* g, const mfvm_ns_params* p, const mfvm_ns_work* w, const mfvm_cnn* net,
+                         const mfvm_closure_work* cw, double cfl, double t_end, int32_t nsteps, int32_t graph,
+                         void* stream) {
+  int rc = check_ns(g, p);
+  if (!rc) rc = check_work(w, MFVM_INT_SSPRK3, true);
+  if (rc) return rc;
+  if (p->dtype != MFVM_F64) return fail(MFVM_E_ARG, "closure_advance: dtype must be f64");
+  if (g->n[2] != 1 || !g->uniform || g->n[0] % 16 != 0)
+    return fail(MFVM_E_UNSUP, "closure_advance: needs a uniform Cartesian 2D grid with nx1 a multiple of 16");
+  for (int v = 0; v < 5; ++v)
+    for (int d = 0; d < 4; ++d)
+      if (p->bc[v][d] != MFVM_BC_PERIODIC) return fail(MFVM_E_UNSUP, "closure_advance: x1 and x2 must be periodic");
+  if (!w->partials || !w->dts) return fail(MFVM_E_ARG, "closure_advance: NULL partials/dts");
+  if (!cw) return fail(MFVM_E_ARG, "closure_advance: NULL work");
+  if (!cw->flag || !cw->counts || !cw->dt_cur || !cw->t_cur || !cw->step)
+    return fail(MFVM_E_ARG, "closure_advance: NULL work buffer");
+  if (net) {
+    for (int l = 0; l < 3; ++l)
+      if (!net->wf[l] || !net->b[l]) return fail(MFVM_E_ARG, "closure_advance: NULL network weights");
+    if (!net->in_shift || !net->in_scale || !net->out_scale) return fail(MFVM_E_ARG, "closure_advance: NULL scales");
+    if (!cw->x || !cw->a1 || !cw->a2 || !cw->y || !cw->T) return fail(MFVM_E_ARG, "closure_advance: NULL work buffer");
+  }
+  if (t_end != t_end) return fail(MFVM_E_ARG, "closure_advance: t_end is NaN");
+  if (nsteps < 0 || graph < 0) return fail(MFVM_E_ARG, "nsteps and graph must be >= 0");
+  if (nsteps == 0) return MFVM_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipError_t e = fill_int(cw->step, 1, 0, s);
+  if (e == hipSuccess) e = fill_int(cw->counts, 3, 0, s);
+  if (e == hipSuccess) e = hipMemsetAsync(cw->t_cur, 0, sizeof(double), s);  // not captured
+  if (e == hipSuccess) e = launch_cfl_partials<double>(g, p, w->U, nullptr, w->partials, s);
+  if (e == hipSuccess) e = launch_dt_append(g, p, w->partials, cfl, t_end, w->dts, cw->dt_cur, cw->t_cur, cw->step, s);
+  if ((rc = hip_check(e, "closure_advance"))) return rc;
+  int done = 0;
+  if (graph > 0) {
+    const int chunk = graph < nsteps ? graph : nsteps;
+    hipGraph_t gr = nullptr;
+    hipGraphExec_t ex = nullptr;
+    if ((rc = hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "closure_advance: capture")))
+      return rc;
+    for (int c = 0; c < chunk && !rc; ++c) rc = closure_step(g, p, w, net, cw, cfl, t_end, s);
+    e = hipStreamEndCapture(s, &gr);
+    if (!rc) rc = hip_check(e, "closure_advance: end capture");
+    if (!rc) rc = hip_check(hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0), "closure_advance: instantiate");
+    for (int r = 0; !rc && r < nsteps / chunk; ++r) {
+      rc = hip_check(hipGraphLaunch(ex, s), "closure_advance: launch");
+      done += chunk;
+    }
+    const hipError_t se = hipStreamSynchronize(s);
+    if (ex) (void)hipGraphExecDestroy(ex);
+    if (gr) (void)hipGraphDestroy(gr);
+    if (rc) return rc;
+    if ((rc = hip_check(se, "closure_advance: sync"))) return rc;
+  }
+  for (; done < nsteps; ++done)
+    if ((rc = closure_step(g, p, w, net, cw, cfl, t_end, s))) return rc;
+  // ghosts as the reference leaves them: the fill of the last stage input
+  return mfvm_apply_bcs(g, 5, &p->bc[0][0], MFVM_F64, w->S2, w->U, stream);
 }
 
 // ---- heat ----

@@ -180,8 +180,8 @@ hipError_t launch_stage(const mfvm_grid* g, const mfvm_ns_params* p, const mfvm_
 }
 
 template <class R>
-hipError_t launch_cfl(const mfvm_grid* g, const mfvm_ns_params* p, const void* U, const void* lo, double cfl,
-                      double* partials, double* dt_out, hipStream_t s) {
+hipError_t launch_cfl_partials(const mfvm_grid* g, const mfvm_ns_params* p, const void* U, const void* lo,
+                               double* partials, hipStream_t s) {
   const Tile t = ns_tile(g, p);
   const Layout lay = make_layout(g->n[0], g->n[1], g->n[2]);
   const Geo geo = make_geo(g);
@@ -195,10 +195,35 @@ hipError_t launch_cfl(const mfvm_grid* g, const mfvm_ns_params* p, const void* U
   else
     hipLaunchKernelGGL((ns_cfl_kernel<false, R>), grd, blk, red, s, lay, geo, q, static_cast<const R*>(U),
                        static_cast<const R*>(lo), partials, t.kz);
-  const long long nblk = (long long)grd.x * grd.y * grd.z;
+  return hipGetLastError();
+}
+
+template <class R>
+hipError_t launch_cfl(const mfvm_grid* g, const mfvm_ns_params* p, const void* U, const void* lo, double cfl,
+                      double* partials, double* dt_out, hipStream_t s) {
+  const hipError_t e = launch_cfl_partials<R>(g, p, U, lo, partials, s);
+  if (e != hipSuccess) return e;
+  const long long nblk = ns_nblocks(g, p);
   hipLaunchKernelGGL(ns_dt_finalize_kernel, dim3(1), dim3(1024), 0, s, partials, nblk, cfl, g->smax,
                      dt_out);
   return hipGetLastError();
+}
+
+// dt of the next step appended to a history: dt = min(cfl dt, t_end - t),
+// hist[*step] = dt, *dt_cur = dt, *t_cur += dt, ++*step (replayable from a
+// captured graph; t_end = +inf: no clipping).
+__global__ void ns_dt_append_kernel(const double* partials, long long nblk, double cfl, double smax, double t_end,
+                                    double* hist, double* dt_cur, double* t_cur, int* step) {
+  double dt = dt_reduce_block(partials, nblk, cfl, smax);
+  if (threadIdx.x == 0) {
+    const double t = *t_cur;
+    if (t_end - t < dt) dt = t_end > t ? t_end - t : 0.0;
+    const int n = *step;
+    hist[n] = dt;
+    *dt_cur = dt;
+    *t_cur = t + dt;
+    *step = n + 1;
+  }
 }
 
 hipError_t launch_dt_finalize(const mfvm_grid* g, const mfvm_ns_params* p, const double* partials,
@@ -209,11 +234,19 @@ hipError_t launch_dt_finalize(const mfvm_grid* g, const mfvm_ns_params* p, const
   return hipGetLastError();
 }
 
+hipError_t launch_dt_append(const mfvm_grid* g, const mfvm_ns_params* p, const double* partials, double cfl,
+                            double t_end, double* hist, double* dt_cur, double* t_cur, int* step, hipStream_t s) {
+  hipLaunchKernelGGL(ns_dt_append_kernel, dim3(1), dim3(1024), 0, s, partials, ns_nblocks(g, p), cfl, g->smax,
+                     t_end, hist, dt_cur, t_cur, step);
+  return hipGetLastError();
+}
+
 template hipError_t launch_closure<double>(const mfvm_grid*, const mfvm_ns_params*, const void*, void*, hipStream_t);
 template hipError_t launch_closure<float>(const mfvm_grid*, const mfvm_ns_params*, const void*, void*, hipStream_t);
 template hipError_t launch_stage<double>(const mfvm_grid*, const mfvm_ns_params*, const mfvm_stage*, hipStream_t);
 template hipError_t launch_stage<float>(const mfvm_grid*, const mfvm_ns_params*, const mfvm_stage*, hipStream_t);
 template hipError_t launch_cfl<double>(const mfvm_grid*, const mfvm_ns_params*, const void*, const void*, double, double*, double*, hipStream_t);
 template hipError_t launch_cfl<float>(const mfvm_grid*, const mfvm_ns_params*, const void*, const void*, double, double*, double*, hipStream_t);
+template hipError_t launch_cfl_partials<double>(const mfvm_grid*, const mfvm_ns_params*, const void*, const void*, double*, hipStream_t);
 
 }  // namespace mfvm

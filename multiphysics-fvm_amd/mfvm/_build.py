@@ -8,7 +8,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(os.path.dirname(PKG), "csrc")
 INCLUDE = os.path.join(os.path.dirname(os.path.dirname(PKG)), "include")
 LIB = os.path.join(PKG, "lib", "libmfvm.so")
-SOURCES = ("capi.hip", "ns.hip", "bc.hip", "heat.hip", "maxwell.hip", "ensemble.hip")
+SOURCES = ("capi.hip", "ns.hip", "bc.hip", "heat.hip", "maxwell.hip", "ensemble.hip", "closure.hip")
 ARCH = os.environ.get("MFVM_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [
@@ -52,7 +52,7 @@ def build(verbose=False, extra=()):
         if verbose and out.strip():
             print(out)
     tmp = LIB + ".tmp"
-    link = ["hipcc", *FLAGS, *extra, *[o for o, _ in jobs], "-o", tmp]
+    link = ["hipcc", *FLAGS, *extra, "-Wl,-z,defs", *[o for o, _ in jobs], "-o", tmp]
     if verbose:
         print(" ".join(link), flush=True)
     rc, out = _run(link)

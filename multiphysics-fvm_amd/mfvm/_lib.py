@@ -93,6 +93,21 @@ class EnsOpts(ctypes.Structure):
                 ("remap", ctypes.c_int32)]
 
 
+CNN_FWD_RELU, CNN_FWD_LINEAR, CNN_BWD_DATA = 0, 1, 2
+
+
+class CNN(ctypes.Structure):
+    _fields_ = [("wf", ctypes.c_void_p * 3), ("b", ctypes.c_void_p * 3), ("in_shift", ctypes.c_void_p),
+                ("in_scale", ctypes.c_void_p), ("out_scale", ctypes.c_void_p)]
+
+
+class ClosureWork(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_void_p), ("a1", ctypes.c_void_p), ("a2", ctypes.c_void_p),
+                ("y", ctypes.c_void_p), ("T", ctypes.c_void_p), ("flag", ctypes.c_void_p),
+                ("counts", ctypes.c_void_p), ("dt_cur", ctypes.c_void_p), ("t_cur", ctypes.c_void_p),
+                ("step", ctypes.c_void_p)]
+
+
 # (name, restype, argtypes) of every exported entry point.
 _P = ctypes.c_void_p
 _G = ctypes.POINTER(Grid)
@@ -120,6 +135,18 @@ SIGNATURES = [
     ("mfvm_ens_advance", ctypes.c_int, [ctypes.POINTER(EnsMember), ctypes.c_int32, _NP,
                                         ctypes.POINTER(EnsOpts), ctypes.c_int32, ctypes.c_int32,
                                         ctypes.c_double, ctypes.c_int32, _P, ctypes.c_int64, _P]),
+    ("mfvm_cnn_conv", ctypes.c_int, [ctypes.c_int32] * 6 + [_P, _P, _P, _P, _P, _P]),
+    ("mfvm_cnn_wgrad", ctypes.c_int, [ctypes.c_int32] * 5 + [_P, _P, _P, _P, _P]),
+    ("mfvm_cnn_pack", ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, _P, _P, _P, _P]),
+    ("mfvm_cnn_adam", ctypes.c_int, [ctypes.c_int64, _P, _P, _P, _P, ctypes.c_float, ctypes.c_float,
+                                     ctypes.c_float, ctypes.c_float, ctypes.c_int32, _P]),
+    ("mfvm_cnn_features", ctypes.c_int, [_G, ctypes.c_int32, ctypes.c_double, ctypes.c_int32,
+                                         ctypes.c_int64, _P, _P, _P, _P, _P]),
+    ("mfvm_cnn_loss", ctypes.c_int, [ctypes.c_int32] * 3 + [ctypes.c_double, ctypes.c_double,
+                                                            _P, _P, _P, _P, _P, _P, _P]),
+    ("mfvm_closure_advance", ctypes.c_int, [_G, _NP, ctypes.POINTER(NSWork), ctypes.POINTER(CNN),
+                                            ctypes.POINTER(ClosureWork), ctypes.c_double,
+                                            ctypes.c_double, ctypes.c_int32, ctypes.c_int32, _P]),
     ("mfvm_heat_rhs", ctypes.c_int, [_G, ctypes.POINTER(HeatParams), _P, _P, _P]),
     ("mfvm_heat_dt", ctypes.c_int, [_G, ctypes.POINTER(HeatParams), ctypes.c_double, _P, _P]),
     ("mfvm_heat_step", ctypes.c_int, [_G, ctypes.POINTER(HeatParams), ctypes.POINTER(NSWork),

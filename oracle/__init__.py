@@ -14,17 +14,19 @@ last bit.  Each function cites the reference file:line it restates.
 How each part is pinned (beyond retyping the formulas):
 
 * limiters, Riemann solvers, diffusive solver -- hand-computed known answers
-  and closed forms (``tests/test_oracle_numerics.py``);
-* boundary conditions and ghost edge/corner averaging -- brute force on tiny
-  arrays (``tests/test_oracle_bcs.py``);
-* grid metrics -- closed forms per coordinate system, volume sums
-  (``tests/test_oracle_grid.py``);
-* integrators -- amplification factors with a mock linear right-hand side
-  (``tests/test_oracle_integrators.py``);
-* the composed right-hand sides, time steps and full steps of the three
-  physics modules -- golden vectors written by the reference itself
-  (``tests/golden/``), plus uniform-state, advection and conservation
-  invariants.
+  and closed forms; boundary conditions and ghost edge/corner averaging --
+  brute force on tiny arrays; grid metrics -- volume sums and the
+  unknown-system error; integrators -- amplification factors with a mock
+  linear right-hand side; uniform-state, advection and closed-form dt checks
+  of the physics modules (``tests/test_oracle_closed_forms.py``);
+* the composed right-hand sides, time steps, grids, ghost fills and full
+  steps of the three physics modules -- golden vectors written by the
+  reference itself (``tests/golden/``, checked in
+  ``tests/test_oracle_golden.py``);
+* the learned flux-correction network (``cnn.py``, no reference
+  counterpart) -- torch CPU autograd in float64 against central finite
+  differences, and the periodic convolution against an explicit loop
+  (``tests/test_oracle_cnn.py``).
 
 Parity unpinned: nothing in this package is left without one of the checks
 above; the spherical/cylindrical metric factors use ``np.sin``/``np.cos``/
