@@ -7,6 +7,10 @@ built library raises immediately, and there is no other compute path.
 import ctypes
 import os
 
+# torch first: its HIP runtime is then the one libmfvm.so binds to (one
+# runtime per process, so device pointers and streams are shared with torch).
+import torch  # noqa: F401
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("MFVM_LIB") or os.path.join(_HERE, "lib", "libmfvm.so")
 

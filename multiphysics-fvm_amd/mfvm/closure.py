@@ -403,3 +403,66 @@ class Trainer:
             hist.append(float(tot) / max(nb, 1))
             log(f"epoch {ep}: loss {hist[-1]:.4f}")
         return hist
+
+
+# ----------------------------------------------------------------------------
+# Evaluation: error against the coarse-grained fine solution, and wall time
+# ----------------------------------------------------------------------------
+
+def _run_to(U, grid, params, schemes, bcs, net, cfl, t_end, graph=50):
+    """Advance U to exactly t_end in chunks sized from the current dt;
+    returns (wall ms, steps taken, stages that fell back)."""
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    t, steps, fb, dt = 0.0, 0, 0, None
+    while t < t_end * (1.0 - 1e-13):
+        n = 1 if dt is None else max(1, int(math.ceil((t_end - t) / dt)))
+        dts, counts, _ = advance(U, grid, params, schemes, bcs, net, cfl, n, t_end=t_end - t,
+                                 graph=graph if n > graph else 0)
+        d = dts.cpu()
+        t += float(d[:n].sum())
+        steps += int((d[:n] > 0).sum())
+        fb += int(counts.sum())
+        dt = float(d[n]) if float(d[n]) > 0 else float(d[:n].max())
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3, steps, fb
+
+
+def evaluate(net, seeds, params, schemes, bcs, n_coarse=128, factor=4, t_end=2.0, cfl=0.4, log=print):
+    """Per-field L1 error at t_end against the 4x block average of the fine
+    plain run, for plain and corrected runs at n_coarse and the plain run at
+    2 n_coarse (block-averaged to n_coarse); plus wall times (graph replay)."""
+    from .grid.geometry import build_grid
+    nf = n_coarse * factor
+
+    def cfg(n):
+        return {"x1_min": 0.0, "x1_max": 1.0, "nx1": n, "x2_min": 0.0, "x2_max": 1.0, "nx2": n,
+                "x3_min": 0.0, "x3_max": 1.0, "nx3": 1}
+
+    grids = {n: build_grid("cartesian", cfg(n)) for n in (n_coarse, 2 * n_coarse, nf)}
+    res = {"plain_c": [], "closure_c": [], "plain_2c": []}
+    wall = {"plain_c": 0.0, "closure_c": 0.0, "plain_2c": 0.0, "plain_f": 0.0}
+    fallbacks = 0
+    for s in seeds:
+        fine = ns.allocate_u(cfg(nf))
+        for k, a in zip(ns.VARS, kh_initial(nf, s)):
+            fine[k][1:-1, 1:-1, 1:-1] = torch.from_numpy(a).to(fine.device)
+        ic = {n: coarse_grain(fine, nf // n) for n in (n_coarse, 2 * n_coarse)}
+        ms, _, _ = _run_to(fine, grids[nf], params, schemes, bcs, None, cfl, t_end)
+        wall["plain_f"] += ms
+        truth = coarse_grain(fine, factor)
+        for key, n, nn in (("plain_c", n_coarse, None), ("closure_c", n_coarse, net), ("plain_2c", 2 * n_coarse, None)):
+            U = state_from(ic[n], cfg(n))
+            ms, _, fb = _run_to(U, grids[n], params, schemes, bcs, nn, cfl, t_end)
+            wall[key] += ms
+            if key == "closure_c":
+                fallbacks += fb
+            got = coarse_grain(U, n // n_coarse)
+            res[key].append((got - truth)[list(FIELDS)].abs().mean(dim=(1, 2)).cpu().numpy())
+        log(f"seed {s}: L1 mass plain {res['plain_c'][-1][0]:.3e} closure {res['closure_c'][-1][0]:.3e} "
+            f"plain2x {res['plain_2c'][-1][0]:.3e}")
+    out = {k: np.mean(np.stack(v), axis=0).tolist() for k, v in res.items()}
+    out["wall_ms_per_run"] = {k: v / len(seeds) for k, v in wall.items()}
+    out["closure_fallback_stages"] = fallbacks
+    out["seeds"] = list(seeds)
+    return out
