@@ -255,3 +255,34 @@ def build_grid(coordinate_system, grid_config, device="cuda"):
                      "x3": L({"ln_hx1hx2hx3": M(_lib.MET_D3H), "ln_hx1": M(_lib.MET_D3H1),
                               "ln_hx2": M(_lib.MET_D3H2)})}},
     }
+
+
+def build_grid_host(coordinate_system, grid_config):
+    """The reference's grid dict (geometry.py:181-192) with NumPy arrays on
+    the host, for host-side code written against the reference (initial
+    conditions, analysis).  Metrics are (a[i]*b[j])*c[k] over the 1D factors,
+    equal to the reference's arrays value for value."""
+    F, _, centres = _factors(coordinate_system, grid_config)
+
+    def M(m):
+        a, b, c = (np.asarray(x, dtype=np.float64) for x in F[m])
+        return (a[:, None, None] * b[None, :, None]) * c[None, None, :]
+
+    X = np.meshgrid(*centres, indexing="ij")
+    return {
+        "config": grid_config,
+        "cell": {"coordinates": {"x1": X[0], "x2": X[1], "x3": X[2]},
+                 "normal": {"x1": np.array([1.0, 0.0, 0.0]), "x2": np.array([0.0, 1.0, 0.0]),
+                            "x3": np.array([0.0, 0.0, 1.0])},
+                 "geometry": {"volume": M(_lib.MET_V),
+                              "surface": {"x1": M(_lib.MET_S1), "x2": M(_lib.MET_S2), "x3": M(_lib.MET_S3)},
+                              "length": {"x1": M(_lib.MET_L1), "x2": M(_lib.MET_L2), "x3": M(_lib.MET_L3)}}},
+        "lame": {"coefficients": {"x1": M(_lib.MET_H1), "x2": M(_lib.MET_H2), "x3": M(_lib.MET_H3)},
+                 "derivatives": {
+                     "x1": {"ln_hx1hx2hx3": M(_lib.MET_D1H), "ln_hx2": M(_lib.MET_D1H2),
+                            "ln_hx3": M(_lib.MET_D1H3)},
+                     "x2": {"ln_hx1hx2hx3": M(_lib.MET_D2H), "ln_hx1": M(_lib.MET_D2H1),
+                            "ln_hx3": M(_lib.MET_D2H3)},
+                     "x3": {"ln_hx1hx2hx3": M(_lib.MET_D3H), "ln_hx1": M(_lib.MET_D3H1),
+                            "ln_hx2": M(_lib.MET_D3H2)}}},
+    }

@@ -7,6 +7,8 @@ and ``advance`` runs a fixed number of steps with dt kept on the device.
 
 import ctypes
 
+import numpy as np
+
 from . import _lib
 from .grid import geometry
 from .grid.boundary_conditions import SIDES
@@ -58,8 +60,27 @@ def _grid_of(U):
 
 
 def apply_ics(U, params, grid, ics):
-    """src/solver.py:64-85."""
-    ics["initializer"](U, params, grid, ics["parameters"])
+    """src/solver.py:64-85.
+
+    Initializers are host code written against NumPy arrays (the reference's
+    setups index, slice and assign U[var] and read grid coordinates with
+    NumPy).  For an mfvm State the initializer therefore runs on a host
+    mirror -- U as NumPy arrays of the reference shape, the grid as
+    :func:`geometry.build_grid_host` -- and the result is copied to the
+    device.  Anything else is passed through unchanged."""
+    if not isinstance(U, State):
+        ics["initializer"](U, params, grid, ics["parameters"])
+        return
+    host = {name: U[name].cpu().numpy().copy() for name in U.names}
+    g = grid.get("mfvm") if isinstance(grid, dict) else None
+    if g is not None:
+        hgrid = geometry.build_grid_host(g.coordinate_system, grid["config"])
+    else:
+        hgrid = grid
+    ics["initializer"](host, params, hgrid, ics["parameters"])
+    import torch
+    for name in U.names:
+        U[name].copy_(torch.from_numpy(np.ascontiguousarray(host[name], dtype=np.float64)))
 
 
 def advance(setup, U, grid, nsteps):
@@ -70,8 +91,27 @@ def advance(setup, U, grid, nsteps):
                            setup["time_config"]["CFL"], nsteps)
 
 
+def _report(setup, U, t, n, t_end):
+    save = setup.get("save_config", {})
+    saver, freq = save.get("saver"), save.get("output_frequency", 0)
+    if freq and n % freq == 0:
+        print(f"n = {n:6d} | t = {t:.2f}/{t_end:.2f}")
+        if saver is not None:
+            saver(U, setup, t, n, save.get("directory"))
+
+
 def run(setup):
-    """src/solver.py:88-151 -- the reference loop (dt read back each step)."""
+    """src/solver.py:88-151.
+
+    With ``time_config["t_end"]`` only, this is the reference loop: dt from
+    ``compute_dt`` (one readback per step), one integrator call, output every
+    ``output_frequency`` steps after the step, labelled with the pre-step time
+    (Q17: ``while t <= t_end``, no clipping).  Two additions: with
+    ``time_config["n_steps"]`` the run takes exactly that many steps with dt
+    kept on the device (chunks between outputs, same dts and results as the
+    step-by-step loop); with a ``closure`` entry (learned flux correction) it
+    advances to exactly ``t_end`` with :func:`mfvm.closure.advance`.
+    Returns U."""
     physics = setup["physics"]
     params = setup["physical_parameters"]
     tc = setup["time_config"]
@@ -80,17 +120,64 @@ def run(setup):
     apply_ics(U, params, grid, setup["initial_conditions"])
     bcs = setup["boundary_conditions"]
     apply_bcs(U, bcs)
+    if "closure" in setup:
+        return _run_closure(setup, U, grid)
     save = setup.get("save_config", {})
-    saver, freq = save.get("saver"), save.get("output_frequency", 0)
+    freq = save.get("output_frequency", 0)
+    if "n_steps" in tc:
+        nsteps = int(tc["n_steps"])
+        t, n = tc.get("t_start", 0.0), 0
+        while n < nsteps:
+            # run up to and including the next output iteration
+            chunk = (min(nsteps, (n + freq - 1) // freq * freq + 1) if freq else nsteps) - n
+            dts = advance(setup, U, grid, chunk).cpu().numpy()
+            last = n + chunk - 1
+            if freq and last % freq == 0:
+                _report(setup, U, t + float(dts[:chunk - 1].sum()), last, tc.get("t_end", float("inf")))
+            t += float(dts[:chunk].sum())
+            n += chunk
+        return U
     t, n = tc["t_start"], 0
     while t <= tc["t_end"]:
         dt = physics.compute_dt(U, params, grid, tc["CFL"])
         setup["time_integrator"](physics.compute_rhs, U, dt, params, grid, setup["space_schemes"],
                                  bcs, apply_bcs)
-        if freq and n % freq == 0:
-            print(f"n = {n:6d} | t = {t:.2f}/{tc['t_end']:.2f} | dt = {dt:.2e} | {100*t/tc['t_end']:.1f}%")
-            if saver is not None:
-                saver(U, setup, t, n, save.get("directory"))
+        _report(setup, U, t, n, tc["t_end"])
         t += dt
         n += 1
     return U
+
+
+def _run_closure(setup, U, grid):
+    from . import closure
+    tc = setup["time_config"]
+    net = closure.FluxCorrectionCNN.load(setup["closure"]["weights"], device=U.device)
+    closure._run_to(U, grid, setup["physical_parameters"], setup["space_schemes"], setup["boundary_conditions"],
+                    net, tc["CFL"], tc["t_end"] - tc.get("t_start", 0.0))
+    save = setup.get("save_config", {})
+    if save.get("saver") is not None:
+        save["saver"](U, setup, tc["t_end"], 0, save.get("directory"))
+    return U
+
+
+def run_ensemble(setups, nsteps=None, compensated=False, autotune=True):
+    """Advance the setups' states together (mfvm.ensemble): one launch per
+    stage phase over all members.  Setups must share physics parameters,
+    schemes and integrator.  Returns (states, dts)."""
+    from .ensemble import Ensemble
+    from .physics import navier_stokes as ns
+    states, grids = [], []
+    for s in setups:
+        g = geometry.build_grid(s["coordinate_system"], s["grid_config"])
+        U = s["physics"].allocate_u(s["grid_config"])
+        apply_ics(U, s["physical_parameters"], g, s["initial_conditions"])
+        states.append(ns.to_compensated(U) if compensated else U)
+        grids.append(g)
+    s0 = setups[0]
+    ens = Ensemble(states, grids, s0["physical_parameters"], s0["space_schemes"],
+                   [s["boundary_conditions"] for s in setups], s0["time_integrator"], compensated=compensated)
+    if autotune:
+        ens.autotune(s0["time_config"]["CFL"])
+    n = nsteps if nsteps is not None else int(s0["time_config"]["n_steps"])
+    dts = ens.advance(s0["time_config"]["CFL"], n)
+    return states, dts
