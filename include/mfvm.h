@@ -154,6 +154,10 @@ typedef struct mfvm_stage {
   void* dinc;                   /* compensated stages: increment (interior) */
   void* snext;                  /* compensated stages: next stage state (ghosted) */
   int32_t comp;                 /* MFVM_COMP_* (requires dtype f32) */
+  void* xy;                     /* optional, 3D stages (f64, or f32 compensated): interior
+                                   5-field scratch of the stage's dtype; the stage then runs as
+                                   two kernels (x1/x2 faces, then x3 faces and the update);
+                                   same results */
 } mfvm_stage;
 
 typedef struct mfvm_heat_params {  /* heat.py: params + space_schemes */
@@ -186,6 +190,8 @@ typedef struct mfvm_ns_work {   /* scratch for whole steps (all physics) */
   void* kacc;                   /* interior, 5 fields */
   double* partials;             /* >= mfvm_ns_partials_size() doubles */
   double* dts;                  /* device: dt of step s written to dts[s] */
+  void* xy;                     /* optional (NS, f64, nx3 > 1): interior 5-field scratch for
+                                   split stages (see mfvm_stage.xy); NULL: fused stages */
 } mfvm_ns_work;
 
 /* ---- library ------------------------------------------------------------------ */
@@ -271,6 +277,8 @@ typedef struct mfvm_ns_comp_work {
   void* kacc;                   /* interior f32 accumulator */
   double* partials;             /* >= mfvm_ns_partials_size() doubles */
   double* dts;                  /* device, nsteps+1 entries */
+  void* xy;                     /* optional (nx3 > 1): interior 5-field f32 scratch for
+                                   split stages; NULL: fused stages */
 } mfvm_ns_comp_work;
 
 int mfvm_ns_comp_advance(const mfvm_grid* g, const mfvm_ns_params* p, const mfvm_ns_comp_work* w,

@@ -155,8 +155,9 @@ static int check_ns(const mfvm_grid* g, const mfvm_ns_params* p) {
   if (p->riemann != MFVM_RS_HLL && p->riemann != MFVM_RS_RUSANOV)
     return fail(MFVM_E_UNSUP, "unsupported Riemann solver");
   const int bx = p->tile[0] > 0 ? p->tile[0] : 64, by = p->tile[1] > 0 ? p->tile[1] : 4;
-  if (bx < 2 || by < 1 || bx * by > 256 || (bx * by) % 64 != 0)
-    return fail(MFVM_E_ARG, "stage tile must have 2 <= x1 threads, x1*x2 threads a multiple of 64 and <= 256");
+  if (bx < 2 || by < 1 || bx * by > 256 || (bx * by) % 64 != 0 || bx + by > bx * by)
+    return fail(MFVM_E_ARG, "stage tile must have 2 <= x1 threads, x1*x2 threads a multiple of 64 and <= 256, "
+                            "and x1 + x2 threads <= x1*x2");
   if (p->tile[2] < 0) return fail(MFVM_E_ARG, "stage tile x3 planes must be >= 0");
   return check_bc(5, &p->bc[0][0]);
 }
@@ -377,6 +378,7 @@ static int check_work(const mfvm_ns_work* w, int32_t integrator, bool need_c) {
 struct NSCtx {
   const mfvm_grid* g;
   const mfvm_ns_params* p;
+  void* xy;  // split-stage scratch or NULL
 };
 static int ns_pre(void* c, const void* in, void* C, void* stream) {
   NSCtx* x = static_cast<NSCtx*>(c);
@@ -398,6 +400,7 @@ static int ns_stage_op(void* c, const void* in, const void* C, const StageIO& io
   st.dt = io.dt;
   st.dt_dev = io.dt_dev;
   st.partials = partials;
+  st.xy = x->xy;
   return mfvm_ns_stage(x->g, x->p, &st, stream);
 }
 
@@ -406,7 +409,7 @@ int mfvm_ns_step(const mfvm_grid* g, const mfvm_ns_params* p, const mfvm_ns_work
   int rc = check_ns(g, p);
   if (!rc) rc = check_work(w, integrator, true);
   if (rc) return rc;
-  NSCtx x{g, p};
+  NSCtx x{g, p, w->xy};
   PhysOps ph{5, p->dtype, &p->bc[0][0], &x, ns_pre, ns_stage_op};
   return step_impl(g, ph, w, integrator, nullptr, dt, nullptr, true, stream);
 }
@@ -420,7 +423,7 @@ int mfvm_ns_advance(const mfvm_grid* g, const mfvm_ns_params* p, const mfvm_ns_w
   if (nsteps < 0) return fail(MFVM_E_ARG, "nsteps must be >= 0");
   if (nsteps == 0) return MFVM_OK;
   if ((rc = mfvm_ns_dt(g, p, w->U, cfl, w->partials, w->dts, stream))) return rc;
-  NSCtx x{g, p};
+  NSCtx x{g, p, w->xy};
   PhysOps ph{5, p->dtype, &p->bc[0][0], &x, ns_pre, ns_stage_op};
   for (int n = 0; n < nsteps; ++n) {
     if ((rc = step_impl(g, ph, w, integrator, w->dts + n, 0.0, w->partials, n == nsteps - 1, stream)))
@@ -503,6 +506,7 @@ int mfvm_ns_comp_advance(const mfvm_grid* g, const mfvm_ns_params* p, const mfvm
       st.dinc = w->dinc;
       st.snext = seq[q].next;
       st.comp = seq[q].comp;
+      st.xy = w->xy;
       if ((rc = mfvm_ns_stage(g, p, &st, stream))) return rc;
     }
     if (integrator == MFVM_INT_EULER && (rc = mfvm_copy_interior(g, 5, MFVM_F32, w->Sa, w->hi, stream)))
@@ -522,8 +526,9 @@ static int check_ens_members(const mfvm_ens_member* m, int32_t n) {
     if (!rc) rc = check_bc(5, &m[i].bc[0][0]);
     if (rc) return rc;
     const int bx = m[i].tile[0] > 0 ? m[i].tile[0] : 64, by = m[i].tile[1] > 0 ? m[i].tile[1] : 4;
-    if (bx < 2 || by < 1 || bx * by != 256 || m[i].tile[2] < 0)
-      return fail(MFVM_E_ARG, "ensemble: member tile must have x1 threads >= 2, x1*x2 threads == 256, x3 planes >= 0");
+    if (bx < 2 || by < 1 || bx * by != 256 || bx + by > 256 || m[i].tile[2] < 0)
+      return fail(MFVM_E_ARG, "ensemble: member tile must have x1 threads >= 2, x1*x2 threads == 256, "
+                              "x1 + x2 threads <= 256, x3 planes >= 0");
   }
   return MFVM_OK;
 }

@@ -1,10 +1,11 @@
 // Shared device helpers for the finite-volume kernels (gfx950).
 //
 // Floating-point contract: every kernel is compiled with -ffp-contract=off and
-// without fast-math, so each +, -, *, / and sqrt is one IEEE-754 operation
-// rounded to nearest, evaluated in the operand order written here.  The FP64
-// path therefore reproduces NumPy's float64 results of the reference
-// expressions bit for bit.  The helpers below restate NumPy's ufunc semantics
+// without fast-math, so in float64 each +, -, *, / and sqrt is one IEEE-754
+// operation rounded to nearest, evaluated in the operand order written here.
+// The FP64 path therefore reproduces NumPy's float64 results of the reference
+// expressions bit for bit.  (The float32 instantiations divide and take square
+// roots with the hardware approximations, see qdiv / qsqrt.)  The helpers below restate NumPy's ufunc semantics
 // (sign, minimum, maximum) including NaN propagation and tie selection.
 #pragma once
 
@@ -27,6 +28,16 @@ template <class R> MFVM_DEV R nsign(R x) {
 // operand propagates, and on a tie the second operand is returned.
 template <class R> MFVM_DEV R nmin(R a, R b) { return (a < b || a != a) ? a : b; }
 template <class R> MFVM_DEV R nmax(R a, R b) { return (a > b || a != a) ? a : b; }
+
+// Division and square root of the stage arithmetic.  float64: IEEE (the
+// bit-exact path).  float32 (the FP32 and FP32-compensated modes, which are
+// held to an error bar, not to bit parity): the hardware reciprocal and
+// square root (about 1 ulp), an eighth of the instructions of the IEEE
+// sequences.
+template <class R> MFVM_DEV R qdiv(R a, R b) { return a / b; }
+template <> MFVM_DEV float qdiv<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+template <class R> MFVM_DEV R qsqrt(R a) { return sqrt(a); }
+template <> MFVM_DEV float qsqrt<float>(float a) { return __builtin_amdgcn_sqrtf(a); }
 
 // Limited slope, reference slope_limiters.py.  LIM: 1 minmod, 2 MC, 3 van Leer.
 //
@@ -52,7 +63,7 @@ template <int LIM, class R> MFVM_DEV R limit(R b, R f) {
     return same ? copysign(least, b) : (opp ? third : zero);
   } else {
     // slope_limiters.py:94-103 (harmonic mean on denom > 0, else 0)
-    const R harm = ((R(2) * b) * f) / (fabs(b) + fabs(f));
+    const R harm = qdiv(((R(2) * b) * f), (fabs(b) + fabs(f)));
     return same ? copysign(harm, b) : zero;
   }
 }
@@ -96,7 +107,7 @@ template <bool UNIF, class R> MFVM_DEV R g_vol(const Geo& g, int i, int j, int k
 // x / L_d at cell (i,j,k) (0-based interior): exact reciprocal multiply when
 // L_d is a uniform power of two (bitwise identical), true division otherwise.
 template <class R> MFVM_DEV R div_len(const Geo& g, int d, R x, R len) {
-  return g.lpow2[d] ? x * R(g.linv[d]) : x / len;
+  return g.lpow2[d] ? x * R(g.linv[d]) : qdiv(x, len);
 }
 
 // Tile coordinates of this workgroup.  The dispatcher hands workgroups to the

@@ -59,6 +59,22 @@ __global__ __launch_bounds__(256, 2) void ns_stage_comp_kernel(StageArgs a) {
   ns_stage_body<LIM, RS, UNIF, float, true>(a, grid_tile(a));
 }
 
+// Split stage (a.xy != NULL): x1/x2 faces, then x3 faces, sources and update.
+template <int LIM, int RS, bool UNIF, class R>
+__global__ __launch_bounds__(256) void ns_xy_kernel(StageArgs a) {
+  ns_xy_body<LIM, RS, UNIF, R>(a, grid_tile(a));
+}
+
+template <int LIM, int RS, bool UNIF>
+__global__ __launch_bounds__(256) void ns_z_kernel(StageArgs a) {
+  ns_stage_body<LIM, RS, UNIF, double, false, true>(a, grid_tile(a));
+}
+
+template <int LIM, int RS, bool UNIF>
+__global__ __launch_bounds__(256) void ns_z_comp_kernel(StageArgs a) {
+  ns_stage_body<LIM, RS, UNIF, float, true, true>(a, grid_tile(a));
+}
+
 // Standalone CFL partials (navier_stokes.py:512-583), same grid as the stage.
 template <bool UNIF, class R>
 __global__ __launch_bounds__(256) void ns_cfl_kernel(Layout lay, Geo geo, NSPar p, const R* U, const R* lo,
@@ -122,6 +138,26 @@ hipError_t launch_closure(const mfvm_grid* g, const mfvm_ns_params* p, const voi
 
 template <int LIM, int RS, class R>
 static hipError_t launch_stage_t(const StageArgs& a, bool unif, dim3 grd, dim3 blk, size_t lds, hipStream_t s) {
+  const bool comp = a.comp != MFVM_COMP_NONE;
+  if (a.xy && comp == (sizeof(R) == sizeof(float))) {
+    // split stage: x1/x2 faces (timed as the stage), then x3 faces, sources,
+    // update and CFL (timed as "other")
+    const size_t red = 2 * sizeof(double) * ((blk.x * blk.y + 63) / 64);
+    const int tx = t_begin(MFVM_TK_STAGE, s);
+    if (unif) hipLaunchKernelGGL((ns_xy_kernel<LIM, RS, true, R>), grd, blk, lds, s, a);
+    else hipLaunchKernelGGL((ns_xy_kernel<LIM, RS, false, R>), grd, blk, lds, s, a);
+    t_end(tx, s);
+    const int tz = t_begin(MFVM_TK_OTHER, s);
+    if (comp) {
+      if (unif) hipLaunchKernelGGL((ns_z_comp_kernel<LIM, RS, true>), grd, blk, red, s, a);
+      else hipLaunchKernelGGL((ns_z_comp_kernel<LIM, RS, false>), grd, blk, red, s, a);
+    } else {
+      if (unif) hipLaunchKernelGGL((ns_z_kernel<LIM, RS, true>), grd, blk, red, s, a);
+      else hipLaunchKernelGGL((ns_z_kernel<LIM, RS, false>), grd, blk, red, s, a);
+    }
+    t_end(tz, s);
+    return hipGetLastError();
+  }
   const int tk = t_begin(MFVM_TK_STAGE, s);
   if (a.comp != MFVM_COMP_NONE) {
     if (unif)
@@ -164,6 +200,7 @@ hipError_t launch_stage(const mfvm_grid* g, const mfvm_ns_params* p, const mfvm_
   a.dinc = st->dinc;
   a.snext = st->snext;
   a.comp = st->comp;
+  a.xy = g->n[2] > 1 ? st->xy : nullptr;
   const dim3 blk(t.bx, t.by, 1);
   const dim3 grd = ns_grid(g, t);
   const size_t lds = stage_lds_bytes(t.bx, t.by, sizeof(R));

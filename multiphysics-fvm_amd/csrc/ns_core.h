@@ -64,6 +64,9 @@ struct StageArgs {
   double* partials;
   int kz;
   int remap;  // XCD-aware tile order (tile_id)
+  // split stages: interior [5] in-plane divergence written by the x1/x2
+  // kernel and read by the x3 kernel (in the stage's precision R)
+  void* xy;
   // compensated FP32 stages (MFVM_COMP_*): lo = compensation of U0 (ghosted),
   // dinc = increment over U0 (interior), snext = next stage state (ghosted)
   void* lo;
@@ -78,7 +81,7 @@ struct StageArgs {
 // :100  P = (ga-1)*rho*(E - (v1**2+v2**2+v3**2)/2), from the cell's U.
 template <class R>
 MFVM_DEV R closure_p(const NSPar& p, R rho, R v1, R v2, R v3, R rhoE) {
-  const R E = rhoE / rho;
+  const R E = qdiv(rhoE, rho);
   return (R(p.gm1) * rho) * (E - (((v1 * v1) + (v2 * v2)) + (v3 * v3)) / R(2));
 }
 
@@ -86,9 +89,9 @@ MFVM_DEV R closure_p(const NSPar& p, R rho, R v1, R v2, R v3, R rhoE) {
 template <class R>
 MFVM_DEV void load_v(const R* __restrict__ U, long long o, long long fs, R& rho, R v[3]) {
   rho = U[o];
-  v[0] = U[o + fs] / rho;
-  v[1] = U[o + 2 * fs] / rho;
-  v[2] = U[o + 3 * fs] / rho;
+  v[0] = qdiv(U[o + fs], rho);
+  v[1] = qdiv(U[o + 2 * fs], rho);
+  v[2] = qdiv(U[o + 3 * fs], rho);
 }
 
 // :111-139 Stokes tensor from the centred differences dv[a][b] = v_a(+b) -
@@ -105,7 +108,7 @@ MFVM_DEV void closure_t(const Geo& geo, const NSPar& p, int i, int j, int k, con
     } else {
       const R twoL = R(2) * g_len<UNIF, R>(geo, b, i, j, k);
 #pragma unroll
-      for (int a = 0; a < 3; ++a) g[a][b] = dv[a][b] / twoL;
+      for (int a = 0; a < 3; ++a) g[a][b] = qdiv(dv[a][b], twoL);
     }
   }
   const R v1 = v[0], v2 = v[1], v3 = v[2];
@@ -326,9 +329,9 @@ MFVM_DEV void cell_states(const StageArgs& a, const R* U, const R* C, int i, int
 template <int D, int RS, class R>
 MFVM_DEV void face_flux(const NSPar& p, const R l[NQ], const R r[NQ], R area, R fsout[5]) {
   const R rl = l[0], rr = r[0];
-  const R vl[3] = {l[1] / rl, l[2] / rl, l[3] / rl};
-  const R vr[3] = {r[1] / rr, r[2] / rr, r[3] / rr};
-  const R El = l[4] / rl, Er = r[4] / rr;
+  const R vl[3] = {qdiv(l[1], rl), qdiv(l[2], rl), qdiv(l[3], rl)};
+  const R vr[3] = {qdiv(r[1], rr), qdiv(r[2], rr), qdiv(r[3], rr)};
+  const R El = qdiv(l[4], rl), Er = qdiv(r[4], rr);
   const R Pl = l[5], Pr = r[5];
   R fl[5], fr[5];
   fl[0] = rl * vl[D];
@@ -345,8 +348,8 @@ MFVM_DEV void face_flux(const NSPar& p, const R l[NQ], const R r[NQ], R area, R 
   fl[4] = ((rl * El) + Pl) * vl[D];
   fr[4] = ((rr * Er) + Pr) * vr[D];
   // :309  cs = sqrt(ga*P/rho)
-  const R csl = sqrt((R(p.gamma) * Pl) / rl);
-  const R csr = sqrt((R(p.gamma) * Pr) / rr);
+  const R csl = qsqrt(qdiv((R(p.gamma) * Pl), rl));
+  const R csr = qsqrt(qdiv((R(p.gamma) * Pr), rr));
   R F[5];
   if constexpr (RS == MFVM_RS_HLL) {
     // Wave set {vn, vn-cs, vn+cs}: its min is vn-cs and its max vn+cs.
@@ -362,7 +365,7 @@ MFVM_DEV void face_flux(const NSPar& p, const R l[NQ], const R r[NQ], R area, R 
       const R den = smax - smin;
       const R sm = smin * smax;
 #pragma unroll
-      for (int q = 0; q < 5; ++q) F[q] = ((smax * fl[q] - smin * fr[q]) + sm * (r[q] - l[q])) / den;
+      for (int q = 0; q < 5; ++q) F[q] = qdiv(((smax * fl[q] - smin * fr[q]) + sm * (r[q] - l[q])), den);
     }
   } else {
     // max |{vn, vn-cs, vn+cs}| is |vn| + cs, rounded once.
@@ -509,7 +512,187 @@ MFVM_DEV TileCtx grid_tile(const StageArgs& a) {
   c.slot = a.partials ? a.partials + 2 * hw_block_id() : nullptr;
   return c;
 }
-template <int LIM, int RS, bool UNIF, class R, bool COMP>
+// x1 and x2 faces of plane m of a tile: states of the own cells and of the
+// tile-edge cells, F.S of every face once, and the in-plane part of the
+// divergence accm = ((F1(i+1/2) - F1(i-1/2)) + F2(j+1/2)) - F2(j-1/2)
+// (navier_stokes.py:705-709).  Every thread of the workgroup must call it (it
+// holds two barriers); cu: the 12 stored quantities of the own cell.
+template <int LIM, int RS, bool UNIF, class R>
+MFVM_DEV void inplane_div(const StageArgs& a, const R* __restrict__ U, const R* __restrict__ C, R* sm, int BX,
+                          int BY, int tx, int ty, int tid, int i0, int j0, int i, int j, bool own, int m,
+                          const R cu[12], R accm[5]) {
+  const int n1 = a.lay.n1, n2 = a.lay.n2;
+  const long long fs = a.lay.fs;
+  const bool visc = a.p.visc != 0;
+  const int wx = BX + 1;
+  const int nrx = BY * wx, nry = (BY + 1) * BX;
+  // LDS: RX[9][BY][BX+1] lower-face states of the x1 faces, LX[9][BY]
+  // upper-face states of the tile-edge cells, RY/LY likewise for x2.  The
+  // face fluxes FX[5][BY][BX+1] / FY[5][BY+1][BX] reuse the RX / RY space
+  // once every state has been read (one more barrier, a third less LDS).
+  R* RX = sm;
+  R* LX = RX + NQ * nrx;
+  R* RY = LX + NQ * BY;
+  R* LY = RY + NQ * nry;
+  R* FX = RX;
+  R* FY = RY;
+  const int NT = BX * BY;
+  __syncthreads();  // the previous plane's reads of FX / FY are done
+  R hx[NQ], hy[NQ];
+  if (j <= n2 && i <= n1 + 1) {
+    R lo[NQ];
+    if (i <= n1) {
+      // own cell: centre from the z-window, neighbours from memory
+      const long long o = a.lay.at(i, j, m);
+      R q[NQ];
+#pragma unroll
+      for (int f = 0; f < 6; ++f) q[f] = cu[f];
+      q[6] = cu[5 + C_T11];
+      q[7] = cu[5 + C_T12];
+      q[8] = cu[5 + C_T13];
+      if (LIM == MFVM_LIM_NONE) {
+#pragma unroll
+        for (int f = 0; f < NQ; ++f) lo[f] = hx[f] = q[f];
+      } else {
+        R qm[NQ], qp[NQ];
+        load_q<0, R>(U, C, fs, o - 1, visc, qm);
+        load_q<0, R>(U, C, fs, o + 1, visc, qp);
+        slope_states<0, LIM, UNIF, R>(a.geo, i, j, m, visc ? 9 : 6, qm, q, qp, lo, hx);
+      }
+    } else {
+      cell_states<0, LIM, UNIF, R>(a, U, C, i, j, m, lo, hx);  // x1 ghost: raw
+    }
+#pragma unroll
+    for (int f = 0; f < NQ; ++f) RX[f * nrx + ty * wx + tx] = lo[f];
+  }
+  if (i <= n1 && j <= n2 + 1) {
+    R lo[NQ];
+    if (j <= n2) {
+      const long long o = a.lay.at(i, j, m);
+      R q[NQ];
+#pragma unroll
+      for (int f = 0; f < 6; ++f) q[f] = cu[f];
+      q[6] = cu[5 + C_T12];
+      q[7] = cu[5 + C_T22];
+      q[8] = cu[5 + C_T23];
+      if (LIM == MFVM_LIM_NONE) {
+#pragma unroll
+        for (int f = 0; f < NQ; ++f) lo[f] = hy[f] = q[f];
+      } else {
+        R qm[NQ], qp[NQ];
+        load_q<1, R>(U, C, fs, o - a.lay.sy, visc, qm);
+        load_q<1, R>(U, C, fs, o + a.lay.sy, visc, qp);
+        slope_states<1, LIM, UNIF, R>(a.geo, i, j, m, visc ? 9 : 6, qm, q, qp, lo, hy);
+      }
+    } else {
+      cell_states<1, LIM, UNIF, R>(a, U, C, i, j, m, lo, hy);  // x2 ghost: raw
+    }
+#pragma unroll
+    for (int f = 0; f < NQ; ++f) RY[f * nry + ty * BX + tx] = lo[f];
+  }
+  // helpers, phase 1: [0, 2BX) x2 tile-edge cells, [2BX, 2BX+2BY) x1 tile-edge cells
+  for (int t = tid; t < 2 * (BX + BY); t += NT) {
+    R lo[NQ], hi[NQ];
+    if (t < 2 * BX) {
+      const int c = t % BX, side = t / BX, ii = i0 + c;
+      if (ii <= n1) {
+        if (side == 0) {
+          cell_states<1, LIM, UNIF, R>(a, U, C, ii, j0 - 1, m, lo, hi);
+#pragma unroll
+          for (int f = 0; f < NQ; ++f) LY[f * BX + c] = hi[f];
+        } else if (j0 + BY <= n2 + 1) {
+          cell_states<1, LIM, UNIF, R>(a, U, C, ii, j0 + BY, m, lo, hi);
+#pragma unroll
+          for (int f = 0; f < NQ; ++f) RY[f * nry + BY * BX + c] = lo[f];
+        }
+      }
+    } else {
+      const int u = t - 2 * BX, r = u % BY, side = u / BY, jj = j0 + r;
+      if (jj <= n2) {
+        if (side == 0) {
+          cell_states<0, LIM, UNIF, R>(a, U, C, i0 - 1, jj, m, lo, hi);
+#pragma unroll
+          for (int f = 0; f < NQ; ++f) LX[f * BY + r] = hi[f];
+        } else if (i0 + BX <= n1 + 1) {
+          cell_states<0, LIM, UNIF, R>(a, U, C, i0 + BX, jj, m, lo, hi);
+#pragma unroll
+          for (int f = 0; f < NQ; ++f) RX[f * nrx + r * wx + BX] = lo[f];
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // phase 2: fluxes into registers (own x1 and x2 faces; one tile-edge face
+  // for each of the last BX + BY threads), then stored over the states
+  R fx[5], fy[5], fh[5];
+  if (own) {
+    R rr[NQ];
+#pragma unroll
+    for (int f = 0; f < NQ; ++f) rr[f] = RX[f * nrx + ty * wx + tx + 1];
+    face_flux<0, RS, R>(a.p, hx, rr, g_area<UNIF, R>(a.geo, 0, i, j - 1, m - 1), fx);
+#pragma unroll
+    for (int f = 0; f < NQ; ++f) rr[f] = RY[f * nry + (ty + 1) * BX + tx];
+    face_flux<1, RS, R>(a.p, hy, rr, g_area<UNIF, R>(a.geo, 1, i - 1, j, m - 1), fy);
+  }
+  // helper faces: u in [0, BX) faces j0-1/2, [BX, BX+BY) faces i0-1/2
+  const int u = NT - 1 - tid;
+  int hkind = 0, hidx = 0;
+  if (u < BX + BY) {
+    R ll[NQ], rr[NQ];
+    if (u < BX) {
+      const int c = u, ii = i0 + c;
+      if (ii <= n1) {
+#pragma unroll
+        for (int f = 0; f < NQ; ++f) {
+          ll[f] = LY[f * BX + c];
+          rr[f] = RY[f * nry + c];
+        }
+        face_flux<1, RS, R>(a.p, ll, rr, g_area<UNIF, R>(a.geo, 1, ii - 1, j0 - 1, m - 1), fh);
+        hkind = 1;
+        hidx = c;
+      }
+    } else {
+      const int r = u - BX, jj = j0 + r;
+      if (jj <= n2) {
+#pragma unroll
+        for (int f = 0; f < NQ; ++f) {
+          ll[f] = LX[f * BY + r];
+          rr[f] = RX[f * nrx + r * wx];
+        }
+        face_flux<0, RS, R>(a.p, ll, rr, g_area<UNIF, R>(a.geo, 0, i0 - 1, jj - 1, m - 1), fh);
+        hkind = 2;
+        hidx = r * wx;
+      }
+    }
+  }
+  __syncthreads();  // every state read: the flux arrays may overwrite them
+  if (own) {
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      FX[q * nrx + ty * wx + tx + 1] = fx[q];
+      FY[q * nry + (ty + 1) * BX + tx] = fy[q];
+    }
+  }
+  if (hkind == 1) {
+#pragma unroll
+    for (int q = 0; q < 5; ++q) FY[q * nry + hidx] = fh[q];
+  } else if (hkind == 2) {
+#pragma unroll
+    for (int q = 0; q < 5; ++q) FX[q * nrx + hidx] = fh[q];
+  }
+  __syncthreads();
+  if (own) {
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      const R ax = FX[q * nrx + ty * wx + tx + 1] - FX[q * nrx + ty * wx + tx];
+      accm[q] = (ax + FY[q * nry + (ty + 1) * BX + tx]) - FY[q * nry + ty * BX + tx];
+    }
+  }
+}
+
+// ZONLY: the in-plane part of the divergence comes from a.xy (written by the
+// x1/x2 kernel of a split stage) instead of being computed here.
+template <int LIM, int RS, bool UNIF, class R, bool COMP, bool ZONLY = false>
 MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl) {
   extern __shared__ double smem_raw[];
   R* sm = reinterpret_cast<R*>(smem_raw);
@@ -526,19 +709,6 @@ MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl) {
   const int ke = min(kb + a.kz, n3 + 1);
   const bool own = (i <= n1) && (j <= n2);
   const bool visc = a.p.visc != 0;
-
-  // LDS carve-up (elements of R): x1 and x2 exchange buffers side by side.
-  // RX[9][BY][BX+1] lower-face states, LX[9][BY] upper-face states of the
-  // x1 tile-edge cells, FX[5][BY][BX+1] face fluxes; RY/LY/FY likewise for x2.
-  const int wx = BX + 1;
-  const int nrx = BY * wx, nry = (BY + 1) * BX;
-  R* RX = sm;
-  R* LX = RX + NQ * nrx;
-  R* FX = LX + NQ * BY;
-  R* RY = FX + 5 * nrx;
-  R* LY = RY + NQ * nry;
-  R* FY = LY + NQ * BX;
-  const int NT = BX * BY;
 
   double dt = a.dt_dev ? *a.dt_dev : a.dt;
 
@@ -611,141 +781,16 @@ MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl) {
     R accm[5], srcm[5];
     bool src_any = false;
     if (m < ke) {
-      R hx[NQ], hy[NQ];
-      if (j <= n2 && i <= n1 + 1) {
-        R lo[NQ];
-        if (i <= n1) {
-          // own cell: centre from the z-window, neighbours from memory
-          const long long o = a.lay.at(i, j, m);
-          R q[NQ];
+      if constexpr (ZONLY) {
+        if (own) {
+          const long long oi = a.il.at(i, j, m);
 #pragma unroll
-          for (int f = 0; f < 6; ++f) q[f] = cu[f];
-          q[6] = cu[5 + C_T11];
-          q[7] = cu[5 + C_T12];
-          q[8] = cu[5 + C_T13];
-          if (LIM == MFVM_LIM_NONE) {
-#pragma unroll
-            for (int f = 0; f < NQ; ++f) lo[f] = hx[f] = q[f];
-          } else {
-            R qm[NQ], qp[NQ];
-            load_q<0, R>(U, C, fs, o - 1, visc, qm);
-            load_q<0, R>(U, C, fs, o + 1, visc, qp);
-            slope_states<0, LIM, UNIF, R>(a.geo, i, j, m, visc ? 9 : 6, qm, q, qp, lo, hx);
-          }
-        } else {
-          cell_states<0, LIM, UNIF, R>(a, U, C, i, j, m, lo, hx);  // x1 ghost: raw
+          for (int q = 0; q < 5; ++q) accm[q] = static_cast<const R*>(a.xy)[oi + q * a.il.fs];
         }
-#pragma unroll
-        for (int f = 0; f < NQ; ++f) RX[f * nrx + ty * wx + tx] = lo[f];
+      } else {
+        inplane_div<LIM, RS, UNIF, R>(a, U, C, sm, BX, BY, tx, ty, tid, i0, j0, i, j, own, m, cu, accm);
       }
-      if (i <= n1 && j <= n2 + 1) {
-        R lo[NQ];
-        if (j <= n2) {
-          const long long o = a.lay.at(i, j, m);
-          R q[NQ];
-#pragma unroll
-          for (int f = 0; f < 6; ++f) q[f] = cu[f];
-          q[6] = cu[5 + C_T12];
-          q[7] = cu[5 + C_T22];
-          q[8] = cu[5 + C_T23];
-          if (LIM == MFVM_LIM_NONE) {
-#pragma unroll
-            for (int f = 0; f < NQ; ++f) lo[f] = hy[f] = q[f];
-          } else {
-            R qm[NQ], qp[NQ];
-            load_q<1, R>(U, C, fs, o - a.lay.sy, visc, qm);
-            load_q<1, R>(U, C, fs, o + a.lay.sy, visc, qp);
-            slope_states<1, LIM, UNIF, R>(a.geo, i, j, m, visc ? 9 : 6, qm, q, qp, lo, hy);
-          }
-        } else {
-          cell_states<1, LIM, UNIF, R>(a, U, C, i, j, m, lo, hy);  // x2 ghost: raw
-        }
-#pragma unroll
-        for (int f = 0; f < NQ; ++f) RY[f * nry + ty * BX + tx] = lo[f];
-      }
-      // helpers, phase 1: [0, 2BX) x2 tile-edge cells, [2BX, 2BX+2BY) x1 tile-edge cells
-      for (int t = tid; t < 2 * (BX + BY); t += NT) {
-        R lo[NQ], hi[NQ];
-        if (t < 2 * BX) {
-          const int c = t % BX, side = t / BX, ii = i0 + c;
-          if (ii <= n1) {
-            if (side == 0) {
-              cell_states<1, LIM, UNIF, R>(a, U, C, ii, j0 - 1, m, lo, hi);
-#pragma unroll
-              for (int f = 0; f < NQ; ++f) LY[f * BX + c] = hi[f];
-            } else if (j0 + BY <= n2 + 1) {
-              cell_states<1, LIM, UNIF, R>(a, U, C, ii, j0 + BY, m, lo, hi);
-#pragma unroll
-              for (int f = 0; f < NQ; ++f) RY[f * nry + BY * BX + c] = lo[f];
-            }
-          }
-        } else {
-          const int u = t - 2 * BX, r = u % BY, side = u / BY, jj = j0 + r;
-          if (jj <= n2) {
-            if (side == 0) {
-              cell_states<0, LIM, UNIF, R>(a, U, C, i0 - 1, jj, m, lo, hi);
-#pragma unroll
-              for (int f = 0; f < NQ; ++f) LX[f * BY + r] = hi[f];
-            } else if (i0 + BX <= n1 + 1) {
-              cell_states<0, LIM, UNIF, R>(a, U, C, i0 + BX, jj, m, lo, hi);
-#pragma unroll
-              for (int f = 0; f < NQ; ++f) RX[f * nrx + r * wx + BX] = lo[f];
-            }
-          }
-        }
-      }
-      __syncthreads();
-      if (own) {
-        R rr[NQ], fo[5];
-#pragma unroll
-        for (int f = 0; f < NQ; ++f) rr[f] = RX[f * nrx + ty * wx + tx + 1];
-        face_flux<0, RS, R>(a.p, hx, rr, g_area<UNIF, R>(a.geo, 0, i, j - 1, m - 1), fo);
-#pragma unroll
-        for (int q = 0; q < 5; ++q) FX[q * nrx + ty * wx + tx + 1] = fo[q];
-#pragma unroll
-        for (int f = 0; f < NQ; ++f) rr[f] = RY[f * nry + (ty + 1) * BX + tx];
-        face_flux<1, RS, R>(a.p, hy, rr, g_area<UNIF, R>(a.geo, 1, i - 1, j, m - 1), fo);
-#pragma unroll
-        for (int q = 0; q < 5; ++q) FY[q * nry + (ty + 1) * BX + tx] = fo[q];
-      }
-      // helpers, phase 2 (last threads): [0, BX) faces j0-1/2, [BX, BX+BY) faces i0-1/2
-      for (int u = NT - 1 - tid; u < BX + BY; u += NT) {
-        R ll[NQ], rr[NQ], fo[5];
-        if (u < BX) {
-          const int c = u, ii = i0 + c;
-          if (ii <= n1) {
-#pragma unroll
-            for (int f = 0; f < NQ; ++f) {
-              ll[f] = LY[f * BX + c];
-              rr[f] = RY[f * nry + c];
-            }
-            face_flux<1, RS, R>(a.p, ll, rr, g_area<UNIF, R>(a.geo, 1, ii - 1, j0 - 1, m - 1), fo);
-#pragma unroll
-            for (int q = 0; q < 5; ++q) FY[q * nry + c] = fo[q];
-          }
-        } else {
-          const int r = u - BX, jj = j0 + r;
-          if (jj <= n2) {
-#pragma unroll
-            for (int f = 0; f < NQ; ++f) {
-              ll[f] = LX[f * BY + r];
-              rr[f] = RX[f * nrx + r * wx];
-            }
-            face_flux<0, RS, R>(a.p, ll, rr, g_area<UNIF, R>(a.geo, 0, i0 - 1, jj - 1, m - 1), fo);
-#pragma unroll
-            for (int q = 0; q < 5; ++q) FX[q * nrx + r * wx] = fo[q];
-          }
-        }
-      }
-      __syncthreads();
-      if (own) {
-#pragma unroll
-        for (int q = 0; q < 5; ++q) {
-          const R ax = FX[q * nrx + ty * wx + tx + 1] - FX[q * nrx + ty * wx + tx];
-          accm[q] = (ax + FY[q * nry + (ty + 1) * BX + tx]) - FY[q * nry + ty * BX + tx];
-        }
-        cell_source<UNIF, R>(a.geo, a.p, i, j, m, cu, cu + 5, srcm, src_any);
-      }
+      if (own) cell_source<UNIF, R>(a.geo, a.p, i, j, m, cu, cu + 5, srcm, src_any);
     }
 
     // ---- finalize cell m-1 -----------------------------------------------------
@@ -758,7 +803,7 @@ MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl) {
 #pragma unroll
       for (int q = 0; q < 5; ++q) {
         const R x = -((acc_prev[q] + fsz[q]) - fsz_prev[q]);
-        kr[q] = vp2 ? x * R(a.geo.vinv) : x / vol;
+        kr[q] = vp2 ? x * R(a.geo.vinv) : qdiv(x, vol);
         if (src_any_prev) kr[q] = kr[q] + src_prev[q];
       }
       const long long oc = a.lay.at(i, j, k);
@@ -857,6 +902,48 @@ MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl) {
   if (tl.slot) block_max2<double>(rate_max, nu_max, tl.slot, reinterpret_cast<double*>(smem_raw));
 }
 
+// x1/x2 half of a split stage: the in-plane divergence of every cell of the
+// tile's planes, written to a.xy (interior layout).  Same arithmetic as the
+// fused body; the x3 half (ns_stage_body with ZONLY) completes the stage.
+template <int LIM, int RS, bool UNIF, class R>
+MFVM_DEV void ns_xy_body(const StageArgs& a, const TileCtx& tl) {
+  extern __shared__ double smem_raw[];
+  R* sm = reinterpret_cast<R*>(smem_raw);
+  const R* __restrict__ U = static_cast<const R*>(a.U);
+  const R* __restrict__ C = static_cast<const R*>(a.C);
+  const int BX = tl.bx, BY = tl.by;
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  const int tx = tid % BX, ty = tid / BX;
+  const int n1 = a.lay.n1, n2 = a.lay.n2, n3 = a.lay.n3;
+  const long long fs = a.lay.fs;
+  const int i0 = 1 + tl.x * BX, j0 = 1 + tl.y * BY;
+  const int i = i0 + tx, j = j0 + ty;
+  const int kb = 1 + tl.z * a.kz;
+  const int ke = min(kb + a.kz, n3 + 1);
+  const bool own = (i <= n1) && (j <= n2);
+  const bool visc = a.p.visc != 0;
+  for (int m = kb; m < ke; ++m) {
+    R cu[12];
+    if (own) {
+      const long long o = a.lay.at(i, j, m);
+#pragma unroll
+      for (int f = 0; f < 5; ++f) cu[f] = U[o + f * fs];
+#pragma unroll
+      for (int f = 0; f < NS_NC; ++f) cu[5 + f] = (f == 0 || visc) ? C[o + f * fs] : R(0);
+    } else {
+#pragma unroll
+      for (int f = 0; f < 12; ++f) cu[f] = R(0);
+    }
+    R accm[5];
+    inplane_div<LIM, RS, UNIF, R>(a, U, C, sm, BX, BY, tx, ty, tid, i0, j0, i, j, own, m, cu, accm);
+    if (own) {
+      const long long oi = a.il.at(i, j, m);
+#pragma unroll
+      for (int q = 0; q < 5; ++q) static_cast<R*>(a.xy)[oi + q * a.il.fs] = accm[q];
+    }
+  }
+}
+
 // CFL maxima of one tile (navier_stokes.py:512-583): rate_c and nu over the
 // tile's cells, reduced to the tile's slot.  lo != NULL: compensated FP32
 // state, rates in FP64 of hi + lo.
@@ -935,7 +1022,7 @@ inline Tile ns_tile(const mfvm_grid* g, const mfvm_ns_params* p) {
 // LDS bytes of the stage body for a bx x by tile.
 inline size_t stage_lds_bytes(int bx, int by, size_t rsize) {
   const size_t nrx = (size_t)by * (bx + 1), nry = (size_t)(by + 1) * bx;
-  size_t lds = (NQ * nrx + NQ * by + 5 * nrx + NQ * nry + NQ * bx + 5 * nry) * rsize;
+  size_t lds = (NQ * nrx + NQ * by + NQ * nry + NQ * bx) * rsize;
   const size_t red = 2 * sizeof(double) * ((bx * by + 63) / 64);
   return lds < red ? red : lds;
 }

@@ -54,13 +54,14 @@ class Stage(ctypes.Structure):
                 ("acc_coef", ctypes.c_double), ("dt", ctypes.c_double),
                 ("dt_dev", ctypes.c_void_p), ("partials", ctypes.c_void_p),
                 ("lo", ctypes.c_void_p), ("dinc", ctypes.c_void_p), ("snext", ctypes.c_void_p),
-                ("comp", ctypes.c_int32)]
+                ("comp", ctypes.c_int32), ("xy", ctypes.c_void_p)]
 
 
 class NSCompWork(ctypes.Structure):
     _fields_ = [("hi", ctypes.c_void_p), ("lo", ctypes.c_void_p), ("Sa", ctypes.c_void_p),
                 ("Sb", ctypes.c_void_p), ("C", ctypes.c_void_p), ("dinc", ctypes.c_void_p),
-                ("kacc", ctypes.c_void_p), ("partials", ctypes.c_void_p), ("dts", ctypes.c_void_p)]
+                ("kacc", ctypes.c_void_p), ("partials", ctypes.c_void_p), ("dts", ctypes.c_void_p),
+                ("xy", ctypes.c_void_p)]
 
 
 class HeatParams(ctypes.Structure):
@@ -79,7 +80,7 @@ class MaxwellParams(ctypes.Structure):
 class NSWork(ctypes.Structure):
     _fields_ = [("U", ctypes.c_void_p), ("S1", ctypes.c_void_p), ("S2", ctypes.c_void_p),
                 ("C", ctypes.c_void_p), ("kacc", ctypes.c_void_p),
-                ("partials", ctypes.c_void_p), ("dts", ctypes.c_void_p)]
+                ("partials", ctypes.c_void_p), ("dts", ctypes.c_void_p), ("xy", ctypes.c_void_p)]
 
 
 ENS_ORDER_MEMBER, ENS_ORDER_COST, ENS_ORDER_INTERLEAVE = 0, 1, 2

@@ -7,6 +7,7 @@ integrators in :mod:`mfvm.numerics.time_integrators` recognise this module's
 """
 
 import ctypes
+import os
 
 import torch
 
@@ -81,6 +82,12 @@ def compute_dt(U, params, grid, cfl):
     return float(dt[0].item())
 
 
+def split_stages():
+    """Whether 3D float64 stages run split in two kernels (x1/x2 faces, then
+    x3 faces and the update; identical results).  MFVM_SPLIT=0/1 overrides."""
+    return os.environ.get("MFVM_SPLIT", "1") != "0"
+
+
 def work(U, nsteps=0):
     """Scratch buffers of whole fused steps (mfvm_ns_work)."""
     n1, n2, n3 = U.n
@@ -90,7 +97,15 @@ def work(U, nsteps=0):
     w.S2 = U.scratch("S2", tuple(U.buf.shape)).data_ptr()
     w.C = _closure_buf(U).data_ptr()
     w.kacc = U.scratch("kacc", (5, n3, n2, n1)).data_ptr()
+    if U.dtype == torch.float64 and n3 > 1 and split_stages():
+        w.xy = _xy_buf(U).data_ptr()
     return w
+
+
+def _xy_buf(U):
+    """Interior scratch [5, nx3, nx2, nx1] of split stages (state dtype)."""
+    n1, n2, n3 = U.n
+    return U.scratch("xy", (5, n3, n2, n1))
 
 
 def fused_step(U, grid, params, space_schemes, bcs, integrator, dt):
@@ -162,6 +177,8 @@ def advance_compensated(U, grid, params, space_schemes, bcs, integrator, cfl, ns
     w.C = _closure_buf(U).data_ptr()
     w.dinc = U.scratch("dinc", (5, n3, n2, n1)).data_ptr()
     w.kacc = U.scratch("kacc", (5, n3, n2, n1)).data_ptr()
+    if n3 > 1 and os.environ.get("MFVM_SPLIT_COMP", "1") != "0":
+        w.xy = _xy_buf(U).data_ptr()
     n = _lib.lib.mfvm_ns_partials_size(g.ptr(), ctypes.byref(p))
     partials = U.scratch64("partials", max(int(n), 1))
     dts = torch.zeros(nsteps + 1, dtype=torch.float64, device=U.device)
