@@ -751,16 +751,24 @@ MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl) {
       zs(cu, zc);
       slope_states<2, LIM, UNIF, R>(a.geo, i, j, kb - 1, visc ? 9 : 6, zm, zp, zc, lo, lz_prev);
     }
+    // The x3-only pass has no barriers between planes: it loads each plane
+    // one iteration ahead, so that the loads overlap the previous plane's work.
+    if (ZONLY && kb + 1 <= n3 + 1) load12(kb + 1, nx);
   }
 
   for (int m = kb; m <= ke; ++m) {
     // ---- x3 face m-1/2 -------------------------------------------------------
     R fsz[5];
     R lz_cur[NQ];
+    R nn[12];  // ZONLY: plane m+2, loaded now for the next iteration
     if (own) {
       R zc[NQ], lo[NQ];
       zs(cu, zc);
-      if (m + 1 <= n3 + 1) load12(m + 1, nx);
+      if constexpr (ZONLY) {
+        if (m + 2 <= n3 + 1 && m + 1 <= ke) load12(m + 2, nn);
+      } else {
+        if (m + 1 <= n3 + 1) load12(m + 1, nx);
+      }
       if (m == n3 + 1 || LIM == MFVM_LIM_NONE) {
 #pragma unroll
         for (int f = 0; f < NQ; ++f) lo[f] = lz_cur[f] = zc[f];
@@ -896,6 +904,10 @@ MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl) {
       zs(cu, zp);
 #pragma unroll
       for (int f = 0; f < 12; ++f) cu[f] = nx[f];
+      if constexpr (ZONLY) {
+#pragma unroll
+        for (int f = 0; f < 12; ++f) nx[f] = nn[f];
+      }
     }
   }
 
