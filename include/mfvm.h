@@ -154,10 +154,10 @@ typedef struct mfvm_stage {
   void* dinc;                   /* compensated stages: increment (interior) */
   void* snext;                  /* compensated stages: next stage state (ghosted) */
   int32_t comp;                 /* MFVM_COMP_* (requires dtype f32) */
-  void* xy;                     /* optional, 3D stages (f64, or f32 compensated): interior
-                                   5-field scratch of the stage's dtype; the stage then runs as
-                                   two kernels (x1/x2 faces, then x3 faces and the update);
-                                   same results */
+  void* xy;                     /* optional (f64, or f32 compensated): interior 5-field
+                                   scratch of the stage's dtype; the stage then runs as two
+                                   kernels (x1/x2 faces, then x3 faces and the update); same
+                                   results */
 } mfvm_stage;
 
 typedef struct mfvm_heat_params {  /* heat.py: params + space_schemes */
@@ -190,8 +190,8 @@ typedef struct mfvm_ns_work {   /* scratch for whole steps (all physics) */
   void* kacc;                   /* interior, 5 fields */
   double* partials;             /* >= mfvm_ns_partials_size() doubles */
   double* dts;                  /* device: dt of step s written to dts[s] */
-  void* xy;                     /* optional (NS, f64, nx3 > 1): interior 5-field scratch for
-                                   split stages (see mfvm_stage.xy); NULL: fused stages */
+  void* xy;                     /* optional (NS, f64): interior 5-field scratch for split
+                                   stages (see mfvm_stage.xy); NULL: fused stages */
 } mfvm_ns_work;
 
 /* ---- library ------------------------------------------------------------------ */

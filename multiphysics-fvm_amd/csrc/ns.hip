@@ -202,7 +202,7 @@ hipError_t launch_stage(const mfvm_grid* g, const mfvm_ns_params* p, const mfvm_
   a.dinc = st->dinc;
   a.snext = st->snext;
   a.comp = st->comp;
-  a.xy = g->n[2] > 1 ? st->xy : nullptr;
+  a.xy = st->xy;
   const dim3 blk(t.bx, t.by, 1);
   const dim3 grd = ns_grid(g, t);
   const size_t lds = stage_lds_bytes(t.bx, t.by, sizeof(R));

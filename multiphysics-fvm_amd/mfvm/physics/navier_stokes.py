@@ -82,6 +82,19 @@ def compute_dt(U, params, grid, cfl):
     return float(dt[0].item())
 
 
+# 2D grids split from this many cells on (smaller ones are launch-bound and
+# lose the extra launch); MFVM_SPLIT_2D=0/1 forces either way.
+SPLIT_2D_MIN_CELLS = 256 * 256
+
+
+def split_2d(U):
+    """Whether a 2D (nx3 == 1) state's stages split as well."""
+    env = os.environ.get("MFVM_SPLIT_2D")
+    if env is not None:
+        return env != "0"
+    return U.n[0] * U.n[1] >= SPLIT_2D_MIN_CELLS
+
+
 def split_stages():
     """Whether 3D float64 stages run split in two kernels (x1/x2 faces, then
     x3 faces and the update; identical results).  MFVM_SPLIT=0/1 overrides."""
@@ -97,7 +110,7 @@ def work(U, nsteps=0):
     w.S2 = U.scratch("S2", tuple(U.buf.shape)).data_ptr()
     w.C = _closure_buf(U).data_ptr()
     w.kacc = U.scratch("kacc", (5, n3, n2, n1)).data_ptr()
-    if U.dtype == torch.float64 and n3 > 1 and split_stages():
+    if U.dtype == torch.float64 and (n3 > 1 or split_2d(U)) and split_stages():
         w.xy = _xy_buf(U).data_ptr()
     return w
 
