@@ -108,3 +108,18 @@ def test_split_stages_equal_fused_stages(comp, monkeypatch):
     if comp:
         shp = tuple(U.buf.shape)
         assert torch.equal(U.scratch("lo", shp), V.scratch("lo", shp))
+
+
+def test_split_stages_equal_fused_stages_2d(monkeypatch):
+    """The same for a 2D grid (split forced below its size threshold)."""
+    from mfvm import _lib
+    ns, grid, U, params, schemes, bcs = _setup((96, 64, 1), "kh")
+    V = ns.allocate_u({"nx1": 96, "nx2": 64, "nx3": 1})
+    V.buf.copy_(U.buf)
+    monkeypatch.setenv("MFVM_SPLIT_2D", "1")
+    d1 = ns.advance(U, grid, params, schemes, bcs, _lib.INT_SSPRK3, 0.4, 8)
+    monkeypatch.setenv("MFVM_SPLIT_2D", "0")
+    d2 = ns.advance(V, grid, params, schemes, bcs, _lib.INT_SSPRK3, 0.4, 8)
+    torch.cuda.synchronize()
+    assert torch.equal(d1, d2)
+    assert torch.equal(U.buf, V.buf)
