@@ -40,8 +40,9 @@ hipError_t launch_bcs(const mfvm_grid* g, int nvar, const int32_t* bc, const voi
   BCArgs a;
   a.lay = make_layout(g->n[0], g->n[1], g->n[2]);
   a.nvar = nvar;
+  a.code[0] = a.code[1] = 0;
   for (int v = 0; v < nvar; ++v)
-    for (int d = 0; d < 6; ++d) a.bc[v][d] = bc[v * 6 + d];
+    for (int d = 0; d < 6; ++d) set_bc_rule(a, v, d, bc[v * 6 + d]);
   const long long nshell = shell_count(g->n[0], g->n[1], g->n[2]);
   const dim3 grd((unsigned)((nshell + 255) / 256), nvar, 1);
   const int tk = t_begin(MFVM_TK_BC, s);

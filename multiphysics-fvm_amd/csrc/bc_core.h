@@ -7,68 +7,63 @@
 
 namespace mfvm {
 
+// Ghost rules packed 2 bits per (field, side): field v < 5 in code[0] at bit
+// 12 v + 2 side, field 5 in code[1].  (Scalars rather than a runtime-indexed
+// array keep the kernel argument out of scratch memory.)
 struct BCArgs {
   Layout lay;
   int nvar;
-  int bc[MFVM_MAX_BCVARS][6];
+  unsigned long long code[2];
 };
 
-// Face-ghost value at ghosted (i,j,k) lying on exactly the ghost layer of
-// `axis` (side 0 = low, 1 = high); the other two indices are interior.
-template <class R>
-MFVM_DEV R face_ghost(const BCArgs& a, const R* f, int rule, int axis, int side, int i, int j, int k) {
-  const int n = axis == 0 ? a.lay.n1 : (axis == 1 ? a.lay.n2 : a.lay.n3);
-  int adj = side == 0 ? 1 : n;       // adjacent interior cell
-  int opp = side == 0 ? n : 1;       // opposite interior cell (periodic)
-  int src = rule == MFVM_BC_PERIODIC ? opp : adj;
-  int ii = i, jj = j, kk = k;
-  if (axis == 0) ii = src;
-  else if (axis == 1) jj = src;
-  else kk = src;
-  const R v = f[a.lay.at(ii, jj, kk)];
+inline void set_bc_rule(BCArgs& a, int var, int side, int rule) {
+  const int w = var < 5 ? 0 : 1, sh = var < 5 ? 12 * var + 2 * side : 2 * side;
+  a.code[w] = (a.code[w] & ~(3ull << sh)) | ((unsigned long long)(rule & 3) << sh);
+}
+
+MFVM_DEV int bc_rule(const BCArgs& a, int var, int side) {
+  return var < 5 ? (int)((a.code[0] >> (12 * var + 2 * side)) & 3ull) : (int)((a.code[1] >> (2 * side)) & 3ull);
+}
+
+// Face-ghost value along axis D at ghosted (i,j,k), whose D index is 0 or
+// n_D + 1 and whose other indices are interior: the adjacent interior cell
+// (neumann), the opposite one (periodic) or minus the adjacent one
+// (reflecting) -- boundary_conditions.py:31, 58, 85.
+template <int D, class R>
+MFVM_DEV R fg(const BCArgs& a, const R* f, int var, int side, int i, int j, int k) {
+  const int rule = bc_rule(a, var, 2 * D + side);
+  const int n = D == 0 ? a.lay.n1 : (D == 1 ? a.lay.n2 : a.lay.n3);
+  const int src = rule == MFVM_BC_PERIODIC ? (side == 0 ? n : 1) : (side == 0 ? 1 : n);
+  if (D == 0) i = src;
+  else if (D == 1) j = src;
+  else k = src;
+  const R v = f[a.lay.at(i, j, k)];
   return rule == MFVM_BC_REFLECTING ? -v : v;
 }
 
-// Value of any ghost-shell cell of field f (solver.py:36-61 semantics).
+// Value of any ghost-shell cell of field f (solver.py:36-61): face ghosts by
+// their side's rule; an edge is the mean of its two neighbouring face ghosts
+// (that of the later axis first); a corner is the mean of its three
+// neighbouring edges.  Edges and corners recompute their face ghosts from
+// the interior, so no second pass is needed.
 template <class R>
 MFVM_DEV R shell_value(const BCArgs& a, int var, const R* f, int i, int j, int k) {
-  const int n[3] = {a.lay.n1, a.lay.n2, a.lay.n3};
-  const int idx[3] = {i, j, k};
-  int gax[3], gside[3], ng = 0;
-  for (int d = 0; d < 3; ++d) {
-    if (idx[d] == 0 || idx[d] == n[d] + 1) {
-      gax[ng] = d;
-      gside[ng] = idx[d] == 0 ? 0 : 1;
-      ++ng;
-    }
+  const int n1 = a.lay.n1, n2 = a.lay.n2, n3 = a.lay.n3;
+  const bool gi = i == 0 || i == n1 + 1, gj = j == 0 || j == n2 + 1, gk = k == 0 || k == n3 + 1;
+  const int si = i == 0 ? 0 : 1, sj = j == 0 ? 0 : 1, sk = k == 0 ? 0 : 1;
+  const int ii = i == 0 ? 1 : n1, jj = j == 0 ? 1 : n2, kk = k == 0 ? 1 : n3;  // one cell inwards
+  if (gi && gj && gk) {
+    const R e0 = R(0.5) * (fg<2, R>(a, f, var, sk, ii, jj, k) + fg<1, R>(a, f, var, sj, ii, j, kk));
+    const R e1 = R(0.5) * (fg<2, R>(a, f, var, sk, ii, jj, k) + fg<0, R>(a, f, var, si, i, jj, kk));
+    const R e2 = R(0.5) * (fg<1, R>(a, f, var, sj, ii, j, kk) + fg<0, R>(a, f, var, si, i, jj, kk));
+    return ((e0 + e1) + e2) / R(3);
   }
-  auto fg = [&](int d, int side, int x, int y, int z) -> R {
-    return face_ghost<R>(a, f, a.bc[var][2 * d + side], d, side, x, y, z);
-  };
-  // move index along axis d one cell inwards
-  auto inward = [&](int d, int* p) { p[d] = (p[d] == 0) ? 1 : n[d]; };
-  if (ng == 1) return fg(gax[0], gside[0], i, j, k);
-  if (ng == 2) {
-    // edge: 0.5*(f[moved along first ghost axis] + f[moved along second])
-    const int p = gax[0], q = gax[1];
-    int a1[3] = {i, j, k}, a2[3] = {i, j, k};
-    inward(p, a1);  // now a face ghost of axis q
-    inward(q, a2);  // now a face ghost of axis p
-    return R(0.5) * (fg(q, gside[1], a1[0], a1[1], a1[2]) + fg(p, gside[0], a2[0], a2[1], a2[2]));
-  }
-  // corner: (e[moved along x1] + e[moved along x2] + e[moved along x3]) / 3
-  R e[3];
-  for (int d = 0; d < 3; ++d) {
-    int c[3] = {i, j, k};
-    inward(d, c);  // an edge ghost whose ghost axes are the other two
-    const int p = d == 0 ? 1 : 0, q = d == 2 ? 1 : 2;
-    int a1[3] = {c[0], c[1], c[2]}, a2[3] = {c[0], c[1], c[2]};
-    inward(p, a1);
-    inward(q, a2);
-    e[d] = R(0.5) * (fg(q, gside[q == 2 ? 2 : (q == 1 ? 1 : 0)], a1[0], a1[1], a1[2]) +
-                     fg(p, gside[p], a2[0], a2[1], a2[2]));
-  }
-  return ((e[0] + e[1]) + e[2]) / R(3);
+  if (gi && gj) return R(0.5) * (fg<1, R>(a, f, var, sj, ii, j, k) + fg<0, R>(a, f, var, si, i, jj, k));
+  if (gi && gk) return R(0.5) * (fg<2, R>(a, f, var, sk, ii, j, k) + fg<0, R>(a, f, var, si, i, j, kk));
+  if (gj && gk) return R(0.5) * (fg<2, R>(a, f, var, sk, i, jj, k) + fg<1, R>(a, f, var, sj, i, j, kk));
+  if (gi) return fg<0, R>(a, f, var, si, i, j, k);
+  if (gj) return fg<1, R>(a, f, var, sj, i, j, k);
+  return fg<2, R>(a, f, var, sk, i, j, k);
 }
 
 // Ghosted (i, j, k) of shell cell t, t in [0, nshell): first the two x1

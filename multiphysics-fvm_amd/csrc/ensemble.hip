@@ -481,8 +481,9 @@ hipError_t ens_advance(const mfvm_ens_member* m, int n, const mfvm_ns_params* p,
     e.geo = make_geo(&mm.grid);
     e.bca.lay = e.lay;
     e.bca.nvar = 5;
+    e.bca.code[0] = e.bca.code[1] = 0;
     for (int v = 0; v < 5; ++v)
-      for (int d = 0; d < 6; ++d) e.bca.bc[v][d] = mm.bc[v][d];
+      for (int d = 0; d < 6; ++d) set_bc_rule(e.bca, v, d, mm.bc[v][d]);
     int nt[3];
     member_tiles(mm, &e.bx, &e.by, &e.kz, nt);
     e.ntx = nt[0];
