@@ -208,16 +208,23 @@ def main():
 
         runc(args.warmup)
         torch.cuda.synchronize()
+        _lib.timing(1)
         ev0.record()
         runc(args.steps)
         ev1.record()
         torch.cuda.synchronize()
         msc = ev0.elapsed_time(ev1)
+        kmc, _ = _lib.timing(2)
+        _lib.timing(0)
         vc = cells * args.steps / (msc / 1e3) / 1e6
         out["fp32_compensated"] = {"value": round(vc, 2), "unit": "Mcell-updates/s",
                                    "ms_per_step": round(msc / args.steps, 4),
                                    "speedup_vs_f64": round(vc / value, 3),
-                                   "bytes_per_cell_step": 240}
+                                   "bytes_per_cell_step": 240,
+                                   "kernel_ms_per_step": {
+                                       name: round(kmc[i] / args.steps, 4) for i, name in
+                                       enumerate(("stage_xy" if split else "stage", "closure", "bc",
+                                                  "stage_z" if split else "other"))}}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

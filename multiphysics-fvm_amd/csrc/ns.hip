@@ -70,10 +70,10 @@ __global__ __launch_bounds__(256) void ns_z_kernel(StageArgs a) {
   ns_stage_body<LIM, RS, UNIF, double, false, true>(a, grid_tile(a));
 }
 
-// Three waves per SIMD (168 VGPRs, a small spill) beat two here: the pass is
-// latency-bound.  (The FP64 x3 pass spills heavily at three and stays at two.)
+// Two waves per SIMD without spills: three waves (168 VGPRs) spill the
+// batched finalize loads to scratch and run slower.
 template <int LIM, int RS, bool UNIF>
-__global__ __launch_bounds__(256, 3) void ns_z_comp_kernel(StageArgs a) {
+__global__ __launch_bounds__(256, 2) void ns_z_comp_kernel(StageArgs a) {
   ns_stage_body<LIM, RS, UNIF, float, true, true>(a, grid_tile(a));
 }
 

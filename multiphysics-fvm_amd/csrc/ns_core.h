@@ -819,24 +819,34 @@ MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl) {
       R* out = static_cast<R*>(a.out);
       R* ka = static_cast<R*>(a.kacc);
       const R* U0 = static_cast<const R*>(a.U0);
+      // The cell's reads of U0, kacc, lo and dinc are all issued before its
+      // first store (the arrays do not alias the outputs), so that they take
+      // one memory round trip instead of one per field.
       if constexpr (!COMP) {
-        R nu_[5];
+        R nu_[5], u0v[5], kav[5];
         const R dtr = R(dt);
+        const bool need0 = a.update == MFVM_UPD_BASE || a.update == MFVM_UPD_SSP || a.update == MFVM_UPD_ACC6;
+        const bool needk = a.update == MFVM_UPD_ACC6 || a.acc == MFVM_ACC_ADD;
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+          u0v[q] = need0 ? U0[oc + q * fs] : R(0);
+          kav[q] = needk ? ka[oi + q * a.il.fs] : R(0);
+        }
 #pragma unroll
         for (int q = 0; q < 5; ++q) {
           const R uq = zp[q];  // stage input U at cell m-1
           R o = uq;
           switch (a.update) {
             case MFVM_UPD_EULER: o = uq + dtr * kr[q]; break;
-            case MFVM_UPD_BASE: o = U0[oc + q * fs] + R(a.c0 * dt) * kr[q]; break;
-            case MFVM_UPD_SSP: o = (R(a.c0) * U0[oc + q * fs]) + (R(a.c1) * (uq + dtr * kr[q])); break;
-            case MFVM_UPD_ACC6: o = U0[oc + q * fs] + R(dt / 6.0) * (ka[oi + q * a.il.fs] + kr[q]); break;
+            case MFVM_UPD_BASE: o = u0v[q] + R(a.c0 * dt) * kr[q]; break;
+            case MFVM_UPD_SSP: o = (R(a.c0) * u0v[q]) + (R(a.c1) * (uq + dtr * kr[q])); break;
+            case MFVM_UPD_ACC6: o = u0v[q] + R(dt / 6.0) * (kav[q] + kr[q]); break;
             default: break;
           }
           nu_[q] = o;
           if (a.update != MFVM_UPD_NONE) out[oc + q * fs] = o;
           if (a.acc == MFVM_ACC_SET) ka[oi + q * a.il.fs] = kr[q];
-          else if (a.acc == MFVM_ACC_ADD) ka[oi + q * a.il.fs] = ka[oi + q * a.il.fs] + R(a.acc_coef) * kr[q];
+          else if (a.acc == MFVM_ACC_ADD) ka[oi + q * a.il.fs] = kav[q] + R(a.acc_coef) * kr[q];
         }
         if (a.partials) {
           R rate, nu;
@@ -852,23 +862,34 @@ MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl) {
         R* dv = static_cast<R*>(a.dinc);
         R* sn = static_cast<R*>(a.snext);
         const bool fin = a.comp >= MFVM_COMP_FINAL_SET;
+        const bool needd = a.comp == MFVM_COMP_SSP || a.comp == MFVM_COMP_FINAL_SSP;
+        const bool needk = a.comp == MFVM_COMP_FINAL_ACC6 || a.acc == MFVM_ACC_ADD;
         double nw[5];
+        R hv[5], lv[5], dvv[5], kav[5];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+          const long long og = oc + q * fs, ii = oi + q * a.il.fs;
+          hv[q] = U0[og];
+          lv[q] = lo[og];
+          dvv[q] = needd ? dv[ii] : R(0);
+          kav[q] = needk ? ka[ii] : R(0);
+        }
 #pragma unroll
         for (int q = 0; q < 5; ++q) {
           const R kq = kr[q];
           const long long og = oc + q * fs, ii = oi + q * a.il.fs;
-          const R h = U0[og], l = lo[og];
+          const R h = hv[q], l = lv[q];
           if (!fin) {
             R d;
             if (a.comp == MFVM_COMP_SET) d = R(a.c0 * dt) * kq;
-            else d = R(a.c0) * (dv[ii] + R(dt) * kq);
+            else d = R(a.c0) * (dvv[q] + R(dt) * kq);
             dv[ii] = d;
             sn[og] = h + (l + d);
           } else {
             double inc;
             if (a.comp == MFVM_COMP_FINAL_SET) inc = (a.c0 * dt) * double(kq);
-            else if (a.comp == MFVM_COMP_FINAL_SSP) inc = a.c0 * (double(dv[ii]) + dt * double(kq));
-            else inc = (dt / 6.0) * (double(ka[ii]) + double(kq));
+            else if (a.comp == MFVM_COMP_FINAL_SSP) inc = a.c0 * (double(dvv[q]) + dt * double(kq));
+            else inc = (dt / 6.0) * (double(kav[q]) + double(kq));
             const double v = (double(h) + double(l)) + inc;
             const R nh = R(v);
             out[og] = nh;
@@ -876,7 +897,7 @@ MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl) {
             nw[q] = v;
           }
           if (a.acc == MFVM_ACC_SET) ka[ii] = kq;
-          else if (a.acc == MFVM_ACC_ADD) ka[ii] = ka[ii] + R(a.acc_coef) * kq;
+          else if (a.acc == MFVM_ACC_ADD) ka[ii] = kav[q] + R(a.acc_coef) * kq;
         }
         if (fin && a.partials) {
           double rate, nu;
