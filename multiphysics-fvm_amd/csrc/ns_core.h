@@ -255,15 +255,53 @@ MFVM_DEV void closure_march_body(const Layout& lay, const Geo& geo, const NSPar&
 // Reconstruction and face flux
 // ---------------------------------------------------------------------------
 
-template <int D, class R>
-MFVM_DEV void load_q(const R* __restrict__ U, const R* __restrict__ C, long long fs, long long o,
-                     bool visc, R q[NQ]) {
+// Reads of the stored quantities at ghosted offset o: u(o, f) of the five
+// conserved fields, c(o, f) of the closure (P and the Stokes tensor), both
+// field-major with a field stride of fs elements.
+template <class R>
+struct PtrSrc {
+  const R* __restrict__ U;
+  const R* __restrict__ C;
+  long long fs;
+  MFVM_DEV R u(long long o, int f) const { return U[o + f * fs]; }
+  MFVM_DEV R c(long long o, int f) const { return C[o + f * fs]; }
+};
+
+// The same reads through buffer resources: one 32-bit byte offset per cell
+// and the field offset as the scalar offset operand, so that the fields of a
+// cell need no per-field address arithmetic and no 64-bit base per field
+// (fewer scalar registers, fewer of them spilled).  Valid while every array
+// is below 4 GiB (buf_src_ok).
+template <class R>
+struct BufSrc {
+  __amdgpu_buffer_rsrc_t U, C;
+  unsigned fsb;  // field stride in bytes
+  MFVM_DEV R ld(__amdgpu_buffer_rsrc_t r, long long o, int f) const {
+    const int vo = (int)((unsigned)o * (unsigned)sizeof(R));
+    const int so = (int)((unsigned)f * fsb);
+    if constexpr (sizeof(R) == 8)
+      return __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0));
+    else
+      return __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0));
+  }
+  MFVM_DEV R u(long long o, int f) const { return ld(U, o, f); }
+  MFVM_DEV R c(long long o, int f) const { return ld(C, o, f); }
+};
+
+// Buffer resource over [p, p + 4 GiB): raw (stride 0), the gfx9 data-format
+// word; the range is not the array's, the callers keep offsets inside it.
+MFVM_DEV __amdgpu_buffer_rsrc_t buf_rsrc(const void* p) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, -1, 0x00020000);
+}
+
+template <int D, class R, class S>
+MFVM_DEV void load_q(const S& src, long long o, bool visc, R q[NQ]) {
 #pragma unroll
-  for (int f = 0; f < 5; ++f) q[f] = U[o + f * fs];
-  q[5] = C[o];
+  for (int f = 0; f < 5; ++f) q[f] = src.u(o, f);
+  q[5] = src.c(o, 0);
   if (visc) {
 #pragma unroll
-    for (int e = 0; e < 3; ++e) q[6 + e] = C[o + trow(D, e) * fs];
+    for (int e = 0; e < 3; ++e) q[6 + e] = src.c(o, trow(D, e));
   } else {
 #pragma unroll
     for (int e = 0; e < 3; ++e) q[6 + e] = R(0);
@@ -301,14 +339,12 @@ MFVM_DEV void slope_states(const Geo& g, int i, int j, int k, int nf, const R qm
 // States of ghosted cell (i,j,k) along D, loading the stencil from memory.
 // hi: state on the cell's upper face (reference fields_L), lo: on its lower
 // face (fields_R).  Ghost cells carry zero slope (reconstructors.py:127-132).
-template <int D, int LIM, bool UNIF, class R>
-MFVM_DEV void cell_states(const StageArgs& a, const R* U, const R* C, int i, int j, int k,
-                          R lo[NQ], R hi[NQ]) {
-  const long long fs = a.lay.fs;
+template <int D, int LIM, bool UNIF, class R, class S>
+MFVM_DEV void cell_states(const StageArgs& a, const S& src, int i, int j, int k, R lo[NQ], R hi[NQ]) {
   const long long st = D == 0 ? 1 : (D == 1 ? a.lay.sy : a.lay.sz);
   const long long o = a.lay.at(i, j, k);
   R q[NQ];
-  load_q<D, R>(U, C, fs, o, a.p.visc, q);
+  load_q<D, R>(src, o, a.p.visc, q);
   const int cd = D == 0 ? i : (D == 1 ? j : k);
   const int nd = D == 0 ? a.lay.n1 : (D == 1 ? a.lay.n2 : a.lay.n3);
   if (LIM == MFVM_LIM_NONE || cd == 0 || cd == nd + 1) {
@@ -317,8 +353,8 @@ MFVM_DEV void cell_states(const StageArgs& a, const R* U, const R* C, int i, int
     return;
   }
   R qm[NQ], qp[NQ];
-  load_q<D, R>(U, C, fs, o - st, a.p.visc, qm);
-  load_q<D, R>(U, C, fs, o + st, a.p.visc, qp);
+  load_q<D, R>(src, o - st, a.p.visc, qm);
+  load_q<D, R>(src, o + st, a.p.visc, qp);
   slope_states<D, LIM, UNIF, R>(a.geo, i, j, k, a.p.visc ? 9 : 6, qm, q, qp, lo, hi);
 }
 
@@ -517,12 +553,10 @@ MFVM_DEV TileCtx grid_tile(const StageArgs& a) {
 // divergence accm = ((F1(i+1/2) - F1(i-1/2)) + F2(j+1/2)) - F2(j-1/2)
 // (navier_stokes.py:705-709).  Every thread of the workgroup must call it (it
 // holds two barriers); cu: the 12 stored quantities of the own cell.
-template <int LIM, int RS, bool UNIF, class R>
-MFVM_DEV void inplane_div(const StageArgs& a, const R* __restrict__ U, const R* __restrict__ C, R* sm, int BX,
-                          int BY, int tx, int ty, int tid, int i0, int j0, int i, int j, bool own, int m,
-                          const R cu[12], R accm[5]) {
+template <int LIM, int RS, bool UNIF, class R, class S>
+MFVM_DEV void inplane_div(const StageArgs& a, const S& src, R* sm, int BX, int BY, int tx, int ty, int tid,
+                          int i0, int j0, int i, int j, bool own, int m, const R cu[12], R accm[5]) {
   const int n1 = a.lay.n1, n2 = a.lay.n2;
-  const long long fs = a.lay.fs;
   const bool visc = a.p.visc != 0;
   const int wx = BX + 1;
   const int nrx = BY * wx, nry = (BY + 1) * BX;
@@ -555,12 +589,12 @@ MFVM_DEV void inplane_div(const StageArgs& a, const R* __restrict__ U, const R* 
         for (int f = 0; f < NQ; ++f) lo[f] = hx[f] = q[f];
       } else {
         R qm[NQ], qp[NQ];
-        load_q<0, R>(U, C, fs, o - 1, visc, qm);
-        load_q<0, R>(U, C, fs, o + 1, visc, qp);
+        load_q<0, R>(src, o - 1, visc, qm);
+        load_q<0, R>(src, o + 1, visc, qp);
         slope_states<0, LIM, UNIF, R>(a.geo, i, j, m, visc ? 9 : 6, qm, q, qp, lo, hx);
       }
     } else {
-      cell_states<0, LIM, UNIF, R>(a, U, C, i, j, m, lo, hx);  // x1 ghost: raw
+      cell_states<0, LIM, UNIF, R>(a, src, i, j, m, lo, hx);  // x1 ghost: raw
     }
 #pragma unroll
     for (int f = 0; f < NQ; ++f) RX[f * nrx + ty * wx + tx] = lo[f];
@@ -580,12 +614,12 @@ MFVM_DEV void inplane_div(const StageArgs& a, const R* __restrict__ U, const R* 
         for (int f = 0; f < NQ; ++f) lo[f] = hy[f] = q[f];
       } else {
         R qm[NQ], qp[NQ];
-        load_q<1, R>(U, C, fs, o - a.lay.sy, visc, qm);
-        load_q<1, R>(U, C, fs, o + a.lay.sy, visc, qp);
+        load_q<1, R>(src, o - a.lay.sy, visc, qm);
+        load_q<1, R>(src, o + a.lay.sy, visc, qp);
         slope_states<1, LIM, UNIF, R>(a.geo, i, j, m, visc ? 9 : 6, qm, q, qp, lo, hy);
       }
     } else {
-      cell_states<1, LIM, UNIF, R>(a, U, C, i, j, m, lo, hy);  // x2 ghost: raw
+      cell_states<1, LIM, UNIF, R>(a, src, i, j, m, lo, hy);  // x2 ghost: raw
     }
 #pragma unroll
     for (int f = 0; f < NQ; ++f) RY[f * nry + ty * BX + tx] = lo[f];
@@ -597,11 +631,11 @@ MFVM_DEV void inplane_div(const StageArgs& a, const R* __restrict__ U, const R* 
       const int c = t % BX, side = t / BX, ii = i0 + c;
       if (ii <= n1) {
         if (side == 0) {
-          cell_states<1, LIM, UNIF, R>(a, U, C, ii, j0 - 1, m, lo, hi);
+          cell_states<1, LIM, UNIF, R>(a, src, ii, j0 - 1, m, lo, hi);
 #pragma unroll
           for (int f = 0; f < NQ; ++f) LY[f * BX + c] = hi[f];
         } else if (j0 + BY <= n2 + 1) {
-          cell_states<1, LIM, UNIF, R>(a, U, C, ii, j0 + BY, m, lo, hi);
+          cell_states<1, LIM, UNIF, R>(a, src, ii, j0 + BY, m, lo, hi);
 #pragma unroll
           for (int f = 0; f < NQ; ++f) RY[f * nry + BY * BX + c] = lo[f];
         }
@@ -610,11 +644,11 @@ MFVM_DEV void inplane_div(const StageArgs& a, const R* __restrict__ U, const R* 
       const int u = t - 2 * BX, r = u % BY, side = u / BY, jj = j0 + r;
       if (jj <= n2) {
         if (side == 0) {
-          cell_states<0, LIM, UNIF, R>(a, U, C, i0 - 1, jj, m, lo, hi);
+          cell_states<0, LIM, UNIF, R>(a, src, i0 - 1, jj, m, lo, hi);
 #pragma unroll
           for (int f = 0; f < NQ; ++f) LX[f * BY + r] = hi[f];
         } else if (i0 + BX <= n1 + 1) {
-          cell_states<0, LIM, UNIF, R>(a, U, C, i0 + BX, jj, m, lo, hi);
+          cell_states<0, LIM, UNIF, R>(a, src, i0 + BX, jj, m, lo, hi);
 #pragma unroll
           for (int f = 0; f < NQ; ++f) RX[f * nrx + r * wx + BX] = lo[f];
         }
@@ -796,7 +830,8 @@ MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl) {
           for (int q = 0; q < 5; ++q) accm[q] = static_cast<const R*>(a.xy)[oi + q * a.il.fs];
         }
       } else {
-        inplane_div<LIM, RS, UNIF, R>(a, U, C, sm, BX, BY, tx, ty, tid, i0, j0, i, j, own, m, cu, accm);
+        inplane_div<LIM, RS, UNIF, R>(a, PtrSrc<R>{U, C, fs}, sm, BX, BY, tx, ty, tid, i0, j0, i, j, own, m, cu,
+                                      accm);
       }
       if (own) cell_source<UNIF, R>(a.geo, a.p, i, j, m, cu, cu + 5, srcm, src_any);
     }
@@ -938,17 +973,14 @@ MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl) {
 // x1/x2 half of a split stage: the in-plane divergence of every cell of the
 // tile's planes, written to a.xy (interior layout).  Same arithmetic as the
 // fused body; the x3 half (ns_stage_body with ZONLY) completes the stage.
-template <int LIM, int RS, bool UNIF, class R>
-MFVM_DEV void ns_xy_body(const StageArgs& a, const TileCtx& tl) {
+template <int LIM, int RS, bool UNIF, class R, class S>
+MFVM_DEV void ns_xy_body(const StageArgs& a, const TileCtx& tl, const S& src) {
   extern __shared__ double smem_raw[];
   R* sm = reinterpret_cast<R*>(smem_raw);
-  const R* __restrict__ U = static_cast<const R*>(a.U);
-  const R* __restrict__ C = static_cast<const R*>(a.C);
   const int BX = tl.bx, BY = tl.by;
   const int tid = threadIdx.y * blockDim.x + threadIdx.x;
   const int tx = tid % BX, ty = tid / BX;
   const int n1 = a.lay.n1, n2 = a.lay.n2, n3 = a.lay.n3;
-  const long long fs = a.lay.fs;
   const int i0 = 1 + tl.x * BX, j0 = 1 + tl.y * BY;
   const int i = i0 + tx, j = j0 + ty;
   const int kb = 1 + tl.z * a.kz;
@@ -960,15 +992,15 @@ MFVM_DEV void ns_xy_body(const StageArgs& a, const TileCtx& tl) {
     if (own) {
       const long long o = a.lay.at(i, j, m);
 #pragma unroll
-      for (int f = 0; f < 5; ++f) cu[f] = U[o + f * fs];
+      for (int f = 0; f < 5; ++f) cu[f] = src.u(o, f);
 #pragma unroll
-      for (int f = 0; f < NS_NC; ++f) cu[5 + f] = (f == 0 || visc) ? C[o + f * fs] : R(0);
+      for (int f = 0; f < NS_NC; ++f) cu[5 + f] = (f == 0 || visc) ? src.c(o, f) : R(0);
     } else {
 #pragma unroll
       for (int f = 0; f < 12; ++f) cu[f] = R(0);
     }
     R accm[5];
-    inplane_div<LIM, RS, UNIF, R>(a, U, C, sm, BX, BY, tx, ty, tid, i0, j0, i, j, own, m, cu, accm);
+    inplane_div<LIM, RS, UNIF, R>(a, src, sm, BX, BY, tx, ty, tid, i0, j0, i, j, own, m, cu, accm);
     if (own) {
       const long long oi = a.il.at(i, j, m);
 #pragma unroll
@@ -1071,6 +1103,11 @@ inline NSPar make_ns_par(const mfvm_ns_params* p) {
   q.visc = p->mu != 0.0;
   q.grav = (p->g[0] != 0.0) || (p->g[1] != 0.0) || (p->g[2] != 0.0);
   return q;
+}
+
+// BufSrc reads are valid: U (5 fields) and C (NS_NC fields) below 4 GiB.
+inline bool buf_src_ok(const Layout& lay, size_t rsize) {
+  return (unsigned long long)NS_NC * (unsigned long long)lay.fs * rsize < (1ull << 32);
 }
 
 // XCD-aware tile order (tile_id); MFVM_XCD_REMAP=0 selects the plain order.
