@@ -114,7 +114,7 @@ MFVM_DEV void ens_stage_loop(const EnsMem* __restrict__ mem, const StageArgs* __
     tc.y = (w.t / e.ntx) % e.nty;
     tc.z = w.t / (e.ntx * e.nty);
     tc.slot = a.partials ? a.partials + 2 * (long long)w.t : nullptr;
-    ns_stage_body<LIM, RS, UNIF, R, COMP>(a, tc);
+    ns_stage_body<LIM, RS, UNIF, R, COMP, false>(a, tc, ptr_src<R>(a));
     __syncthreads();
   }
 }

@@ -50,13 +50,13 @@ __global__ __launch_bounds__(256) void ns_closure_visc_kernel(Layout lay, Geo ge
 
 template <int LIM, int RS, bool UNIF, class R>
 __global__ __launch_bounds__(256) void ns_stage_kernel(StageArgs a) {
-  ns_stage_body<LIM, RS, UNIF, R, false>(a, grid_tile(a));
+  ns_stage_body<LIM, RS, UNIF, R, false, false>(a, grid_tile(a), ptr_src<R>(a));
 }
 
 // Compensated FP32 stage: half the registers of the FP64 kernel, two waves per SIMD.
 template <int LIM, int RS, bool UNIF>
 __global__ __launch_bounds__(256, 2) void ns_stage_comp_kernel(StageArgs a) {
-  ns_stage_body<LIM, RS, UNIF, float, true>(a, grid_tile(a));
+  ns_stage_body<LIM, RS, UNIF, float, true, false>(a, grid_tile(a), ptr_src<float>(a));
 }
 
 // Split stage (a.xy != NULL): x1/x2 faces, then x3 faces, sources and update.
@@ -71,24 +71,23 @@ __global__ __launch_bounds__(256, FIXED && sizeof(R) == 8 ? 3 : 1) void ns_xy_ke
   if constexpr (FIXED) {
     t.bx = 64;
     t.by = 4;
-    const BufSrc<R> src{buf_rsrc(a.U), buf_rsrc(a.C), (unsigned)(a.lay.fs * (long long)sizeof(R))};
-    ns_xy_body<LIM, RS, UNIF, R>(a, t, src);
+    ns_xy_body<LIM, RS, UNIF, R>(a, t, buf_src<R>(a));
   } else {
-    const PtrSrc<R> src{static_cast<const R*>(a.U), static_cast<const R*>(a.C), a.lay.fs};
-    ns_xy_body<LIM, RS, UNIF, R>(a, t, src);
+    ns_xy_body<LIM, RS, UNIF, R>(a, t, ptr_src<R>(a));
   }
 }
 
+// (Buffer-resource reads measure the same here as pointer reads.)
 template <int LIM, int RS, bool UNIF>
 __global__ __launch_bounds__(256) void ns_z_kernel(StageArgs a) {
-  ns_stage_body<LIM, RS, UNIF, double, false, true>(a, grid_tile(a));
+  ns_stage_body<LIM, RS, UNIF, double, false, true>(a, grid_tile(a), ptr_src<double>(a));
 }
 
 // Two waves per SIMD without spills: three waves (168 VGPRs) spill the
 // batched finalize loads to scratch and run slower.
 template <int LIM, int RS, bool UNIF>
 __global__ __launch_bounds__(256, 2) void ns_z_comp_kernel(StageArgs a) {
-  ns_stage_body<LIM, RS, UNIF, float, true, true>(a, grid_tile(a));
+  ns_stage_body<LIM, RS, UNIF, float, true, true>(a, grid_tile(a), ptr_src<float>(a));
 }
 
 // Standalone CFL partials (navier_stokes.py:512-583), same grid as the stage.

@@ -288,10 +288,20 @@ struct BufSrc {
   MFVM_DEV R c(long long o, int f) const { return ld(C, o, f); }
 };
 
+template <class R>
+MFVM_DEV PtrSrc<R> ptr_src(const StageArgs& a) {
+  return PtrSrc<R>{static_cast<const R*>(a.U), static_cast<const R*>(a.C), a.lay.fs};
+}
+
 // Buffer resource over [p, p + 4 GiB): raw (stride 0), the gfx9 data-format
 // word; the range is not the array's, the callers keep offsets inside it.
 MFVM_DEV __amdgpu_buffer_rsrc_t buf_rsrc(const void* p) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, -1, 0x00020000);
+}
+
+template <class R>
+MFVM_DEV BufSrc<R> buf_src(const StageArgs& a) {
+  return BufSrc<R>{buf_rsrc(a.U), buf_rsrc(a.C), (unsigned)(a.lay.fs * (long long)sizeof(R))};
 }
 
 template <int D, class R, class S>
@@ -725,13 +735,12 @@ MFVM_DEV void inplane_div(const StageArgs& a, const S& src, R* sm, int BX, int B
 }
 
 // ZONLY: the in-plane part of the divergence comes from a.xy (written by the
-// x1/x2 kernel of a split stage) instead of being computed here.
-template <int LIM, int RS, bool UNIF, class R, bool COMP, bool ZONLY = false>
-MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl) {
+// x1/x2 kernel of a split stage) instead of being computed here.  src reads
+// a.U and a.C (PtrSrc or BufSrc).
+template <int LIM, int RS, bool UNIF, class R, bool COMP, bool ZONLY, class S>
+MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl, const S& src) {
   extern __shared__ double smem_raw[];
   R* sm = reinterpret_cast<R*>(smem_raw);
-  const R* __restrict__ U = static_cast<const R*>(a.U);
-  const R* __restrict__ C = static_cast<const R*>(a.C);
   const int BX = tl.bx, BY = tl.by;
   const int tid = threadIdx.y * blockDim.x + threadIdx.x;
   const int tx = tid % BX, ty = tid / BX;
@@ -757,9 +766,9 @@ MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl) {
   auto load12 = [&](int k, R* w) {
     const long long o = a.lay.at(i, j, k);
 #pragma unroll
-    for (int f = 0; f < 5; ++f) w[f] = U[o + f * fs];
+    for (int f = 0; f < 5; ++f) w[f] = src.u(o, f);
 #pragma unroll
-    for (int f = 0; f < NS_NC; ++f) w[5 + f] = (f == 0 || visc) ? C[o + f * fs] : R(0);
+    for (int f = 0; f < NS_NC; ++f) w[5 + f] = (f == 0 || visc) ? src.c(o, f) : R(0);
   };
   // ZS of a 12-vector: U, P, T13, T23, T33
   auto zs = [&](const R* w, R* z) {
@@ -830,8 +839,7 @@ MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl) {
           for (int q = 0; q < 5; ++q) accm[q] = static_cast<const R*>(a.xy)[oi + q * a.il.fs];
         }
       } else {
-        inplane_div<LIM, RS, UNIF, R>(a, PtrSrc<R>{U, C, fs}, sm, BX, BY, tx, ty, tid, i0, j0, i, j, own, m, cu,
-                                      accm);
+        inplane_div<LIM, RS, UNIF, R>(a, src, sm, BX, BY, tx, ty, tid, i0, j0, i, j, own, m, cu, accm);
       }
       if (own) cell_source<UNIF, R>(a.geo, a.p, i, j, m, cu, cu + 5, srcm, src_any);
     }
