@@ -151,6 +151,10 @@ hipError_t launch_closure(const mfvm_grid* g, const mfvm_ns_params* p, const voi
   return hipGetLastError();
 }
 
+// Planes per workgroup of the split x1/x2 pass (measured best at 256^3: 8
+// against 16 and 32).
+constexpr int XY_KZ = 8;
+
 template <int LIM, int RS, class R>
 static hipError_t launch_stage_t(const StageArgs& a, bool unif, dim3 grd, dim3 blk, size_t lds, hipStream_t s) {
   const bool comp = a.comp != MFVM_COMP_NONE;
@@ -158,12 +162,20 @@ static hipError_t launch_stage_t(const StageArgs& a, bool unif, dim3 grd, dim3 b
     // split stage: x1/x2 faces (timed as the stage), then x3 faces, sources,
     // update and CFL (timed as "other")
     const size_t red = 2 * sizeof(double) * ((blk.x * blk.y + 63) / 64);
+    // The x1/x2 pass keeps nothing from one plane to the next, so its
+    // workgroups take short chunks of XY_KZ planes (more workgroups, a
+    // shorter tail); the x3 march keeps the longer chunk, which amortises
+    // its start-up planes.
+    StageArgs ax = a;
+    ax.kz = a.kz < XY_KZ ? a.kz : XY_KZ;
+    ax.partials = nullptr;
+    const dim3 gx(grd.x, grd.y, (a.lay.n3 + ax.kz - 1) / ax.kz);
     const bool fixed = blk.x == 64 && blk.y == 4 && buf_src_ok(a.lay, sizeof(R));
     const int tx = t_begin(MFVM_TK_STAGE, s);
-    if (unif && fixed) hipLaunchKernelGGL((ns_xy_kernel<LIM, RS, true, R, true>), grd, blk, lds, s, a);
-    else if (unif) hipLaunchKernelGGL((ns_xy_kernel<LIM, RS, true, R, false>), grd, blk, lds, s, a);
-    else if (fixed) hipLaunchKernelGGL((ns_xy_kernel<LIM, RS, false, R, true>), grd, blk, lds, s, a);
-    else hipLaunchKernelGGL((ns_xy_kernel<LIM, RS, false, R, false>), grd, blk, lds, s, a);
+    if (unif && fixed) hipLaunchKernelGGL((ns_xy_kernel<LIM, RS, true, R, true>), gx, blk, lds, s, ax);
+    else if (unif) hipLaunchKernelGGL((ns_xy_kernel<LIM, RS, true, R, false>), gx, blk, lds, s, ax);
+    else if (fixed) hipLaunchKernelGGL((ns_xy_kernel<LIM, RS, false, R, true>), gx, blk, lds, s, ax);
+    else hipLaunchKernelGGL((ns_xy_kernel<LIM, RS, false, R, false>), gx, blk, lds, s, ax);
     t_end(tx, s);
     const int tz = t_begin(MFVM_TK_OTHER, s);
     if (comp) {
