@@ -246,7 +246,7 @@ def ensemble(args, torch):
         ms = _time(lambda: ens.advance(0.4, steps), torch)
         out[leg] = {"ms": round(ms, 2), "Mcell_updates_per_s": round(cells * steps / ms * 1e-3, 1),
                     "autotune_s": round(tune_s, 2), "trials": len(ens.tuning_log),
-                    "options": {"order": best["order"], "persist": best["persist"],
+                    "options": {"split": best["split"], "order": best["order"], "persist": best["persist"],
                                 "remap": best["remap"], "graph": best["graph"],
                                 "tiles": {"x".join(map(str, k)): list(v) for k, v in best["tiles"].items()}}}
         del ens, S

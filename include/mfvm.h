@@ -310,6 +310,9 @@ typedef struct mfvm_ens_member {
   void* dinc;                   /* compensated only: interior f32 increment */
   double* partials;             /* >= 2 * mfvm_ens_member_tiles() doubles */
   double* dts;                  /* device, nsteps+1 entries: dt of every step */
+  void* xy;                     /* interior 5-field scratch in the stage dtype: the stage runs
+                                   split (x1/x2 pass, then x3 pass) as in mfvm_stage.xy; NULL
+                                   in any member: every member takes the fused stage */
 } mfvm_ens_member;
 
 typedef struct mfvm_ens_opts {

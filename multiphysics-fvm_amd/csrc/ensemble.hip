@@ -4,10 +4,10 @@
 // (the same device bodies: closure_march_body, ns_stage_body, cfl_tile_body,
 // shell_value), so each member's result equals its own mfvm_ns_advance /
 // mfvm_ns_comp_advance run bit for bit.  What changes is the launch shape:
-// each phase of a stage (ghost fill, closure, fused stage) is ONE launch over
-// a work list holding the tiles of all members, so a step costs the same
-// number of launches for 64 members as for one, and small members no longer
-// leave the GPU idle.  The work list order, the stage tile of each member and
+// each phase of a stage (ghost fill, closure, the fused stage or the split
+// x1/x2 and x3 passes) is ONE launch over a work list holding the tiles of
+// all members, so a step costs the same number of launches for 64 members as
+// for one, and small members no longer leave the GPU idle.  The work list order, the stage tile of each member and
 // persistent vs one-workgroup-per-tile launches are options (picked by the
 // autotuner in mfvm.ensemble); the steps can be captured into a hipGraph and
 // replayed.
@@ -133,6 +133,64 @@ __global__ __launch_bounds__(kEnsBlock, 2) void ens_stage_comp_kernel(const EnsM
                                                                       const EnsWork* __restrict__ work,
                                                                       long long nwork, int remap) {
   ens_stage_loop<LIM, RS, UNIF, float, true>(mem, st, work, nwork, remap);
+}
+
+// Split stage over the work list: the x1/x2 pass of every member's tiles,
+// then the x3 pass (as ns_xy_kernel / ns_z_kernel of the single-grid path).
+template <int LIM, int RS, bool UNIF, class R>
+__global__ __launch_bounds__(kEnsBlock) void ens_xy_kernel(const EnsMem* __restrict__ mem,
+                                                           const StageArgs* __restrict__ st,
+                                                           const EnsWork* __restrict__ work, long long nwork,
+                                                           int remap) {
+  for (long long v = blockIdx.x; v < nwork; v += gridDim.x) {
+    const EnsWork w = work[ens_item(v, nwork, remap != 0)];
+    const EnsMem& e = mem[w.m];
+    const StageArgs& a = st[w.m];
+    TileCtx tc;
+    tc.bx = e.bx;
+    tc.by = e.by;
+    tc.x = w.t % e.ntx;
+    tc.y = (w.t / e.ntx) % e.nty;
+    tc.z = w.t / (e.ntx * e.nty);
+    tc.slot = nullptr;
+    ns_xy_body<LIM, RS, UNIF, R>(a, tc, ptr_src<R>(a));
+    __syncthreads();
+  }
+}
+
+template <int LIM, int RS, bool UNIF, class R, bool COMP>
+MFVM_DEV void ens_z_loop(const EnsMem* __restrict__ mem, const StageArgs* __restrict__ st,
+                         const EnsWork* __restrict__ work, long long nwork, int remap) {
+  for (long long v = blockIdx.x; v < nwork; v += gridDim.x) {
+    const EnsWork w = work[ens_item(v, nwork, remap != 0)];
+    const EnsMem& e = mem[w.m];
+    const StageArgs& a = st[w.m];
+    TileCtx tc;
+    tc.bx = e.bx;
+    tc.by = e.by;
+    tc.x = w.t % e.ntx;
+    tc.y = (w.t / e.ntx) % e.nty;
+    tc.z = w.t / (e.ntx * e.nty);
+    tc.slot = a.partials ? a.partials + 2 * (long long)w.t : nullptr;
+    ns_stage_body<LIM, RS, UNIF, R, COMP, true>(a, tc, ptr_src<R>(a));
+    __syncthreads();
+  }
+}
+
+template <int LIM, int RS, bool UNIF>
+__global__ __launch_bounds__(kEnsBlock) void ens_z_kernel(const EnsMem* __restrict__ mem,
+                                                          const StageArgs* __restrict__ st,
+                                                          const EnsWork* __restrict__ work, long long nwork,
+                                                          int remap) {
+  ens_z_loop<LIM, RS, UNIF, double, false>(mem, st, work, nwork, remap);
+}
+
+template <int LIM, int RS, bool UNIF>
+__global__ __launch_bounds__(kEnsBlock, 2) void ens_z_comp_kernel(const EnsMem* __restrict__ mem,
+                                                                  const StageArgs* __restrict__ st,
+                                                                  const EnsWork* __restrict__ work,
+                                                                  long long nwork, int remap) {
+  ens_z_loop<LIM, RS, UNIF, float, true>(mem, st, work, nwork, remap);
 }
 
 template <bool UNIF, class R>
@@ -336,6 +394,7 @@ struct EnsRun {
   size_t lds_stage;
   int persist_blocks;  // 0: one workgroup per item
   int remap;
+  bool split;          // every member has an xy scratch: x1/x2 pass + x3 pass
   int in_sel[4];
 };
 
@@ -347,6 +406,20 @@ static unsigned grid_for(long long nitems, int persist_blocks) {
 template <int LIM, int RS, bool UNIF>
 static void launch_ens_stage_t(const EnsRun& r, const StageArgs* st, hipStream_t s) {
   const unsigned g = grid_for(r.nsw, r.persist_blocks);
+  if (r.split) {
+    if (r.comp) {
+      hipLaunchKernelGGL((ens_xy_kernel<LIM, RS, UNIF, float>), dim3(g), dim3(kEnsBlock), r.lds_stage, s, r.d_mem,
+                         st, r.d_sw, r.nsw, r.remap);
+      hipLaunchKernelGGL((ens_z_comp_kernel<LIM, RS, UNIF>), dim3(g), dim3(kEnsBlock), r.lds_stage, s, r.d_mem,
+                         st, r.d_sw, r.nsw, r.remap);
+    } else {
+      hipLaunchKernelGGL((ens_xy_kernel<LIM, RS, UNIF, double>), dim3(g), dim3(kEnsBlock), r.lds_stage, s,
+                         r.d_mem, st, r.d_sw, r.nsw, r.remap);
+      hipLaunchKernelGGL((ens_z_kernel<LIM, RS, UNIF>), dim3(g), dim3(kEnsBlock), r.lds_stage, s, r.d_mem, st,
+                         r.d_sw, r.nsw, r.remap);
+    }
+    return;
+  }
   if (r.comp)
     hipLaunchKernelGGL((ens_stage_comp_kernel<LIM, RS, UNIF>), dim3(g), dim3(kEnsBlock), r.lds_stage, s, r.d_mem,
                        st, r.d_sw, r.nsw, r.remap);
@@ -461,6 +534,8 @@ hipError_t ens_advance(const mfvm_ens_member* m, int n, const mfvm_ns_params* p,
   r.nsw = L.nsw;
   r.ncw = L.ncw;
   r.remap = o->remap ? 1 : 0;
+  r.split = true;
+  for (int i = 0; i < n; ++i) r.split = r.split && m[i].xy != nullptr;
   for (int q = 0; q < nst; ++q) r.in_sel[q] = seq[q].in;
   double* d_dt = reinterpret_cast<double*>(base + L.off_dt);
   int* d_step = reinterpret_cast<int*>(base + L.off_step);
@@ -536,6 +611,7 @@ hipError_t ens_advance(const mfvm_ens_member* m, int n, const mfvm_ns_params* p,
       a.kz = e.kz;
       a.remap = 0;
       a.comp = seq[q].comp;
+      a.xy = r.split ? mm.xy : nullptr;
       if (comp) {
         const bool fin = seq[q].comp >= MFVM_COMP_FINAL_SET;
         a.out = fin ? mm.U : nullptr;

@@ -90,7 +90,8 @@ class EnsMember(ctypes.Structure):
     _fields_ = [("grid", Grid), ("bc", (ctypes.c_int32 * 6) * 5), ("tile", ctypes.c_int32 * 3),
                 ("U", ctypes.c_void_p), ("S1", ctypes.c_void_p), ("S2", ctypes.c_void_p),
                 ("C", ctypes.c_void_p), ("kacc", ctypes.c_void_p), ("lo", ctypes.c_void_p),
-                ("dinc", ctypes.c_void_p), ("partials", ctypes.c_void_p), ("dts", ctypes.c_void_p)]
+                ("dinc", ctypes.c_void_p), ("partials", ctypes.c_void_p), ("dts", ctypes.c_void_p),
+                ("xy", ctypes.c_void_p)]
 
 
 class EnsOpts(ctypes.Structure):

@@ -27,7 +27,7 @@ from .state import State
 ORDERS = {"member": _lib.ENS_ORDER_MEMBER, "cost": _lib.ENS_ORDER_COST,
           "interleave": _lib.ENS_ORDER_INTERLEAVE}
 
-DEFAULT_OPTIONS = {"tiles": {}, "order": "cost", "persist": 0, "graph": 50, "remap": 1}
+DEFAULT_OPTIONS = {"tiles": {}, "order": "cost", "persist": 0, "graph": 50, "remap": 1, "split": 1}
 
 # Stage tile candidates (x1 threads, x2 threads, x3 planes); x1*x2 == 256.
 TILES_2D = ((64, 4, 1), (32, 8, 1), (16, 16, 1), (128, 2, 1))
@@ -70,7 +70,7 @@ class Ensemble:
         self._stream = None
 
     # ---- members -------------------------------------------------------------
-    def _member(self, U, g, bcs, tile):
+    def _member(self, U, g, bcs, tile, split):
         m = _lib.EnsMember()
         m.grid = g.desc
         rows, _ = c.bc_table(ns.VARS, bcs)
@@ -90,6 +90,8 @@ class Ensemble:
         if ntiles < 0:
             _lib.check(_lib.E_ARG)
         m.partials = U.scratch64("ens_partials", 2 * int(ntiles)).data_ptr()
+        # split stage: x1/x2 pass into this interior scratch, then the x3 pass
+        m.xy = U.scratch("xy", (5, n3, n2, n1)).data_ptr() if split else None
         return m
 
     def _tile_of(self, U, tiles):
@@ -100,7 +102,8 @@ class Ensemble:
         arr = (_lib.EnsMember * len(states))()
         dts = torch.zeros((len(states), nsteps + 1), dtype=torch.float64, device=states[0].device)
         for i, U in enumerate(states):
-            arr[i] = self._member(U, self.grids[i], self.bcs[i], self._tile_of(U, tiles))
+            arr[i] = self._member(U, self.grids[i], self.bcs[i], self._tile_of(U, tiles),
+                                  bool(options.get("split", 1)))
             arr[i].dts = dts[i].data_ptr()
         need = _lib.lib.mfvm_ens_table_bytes(arr, len(states), self.integrator)
         if need < 0:
@@ -156,9 +159,10 @@ class Ensemble:
         return out
 
     def autotune(self, cfl, steps=3, verbose=False):
-        """Pick tiles per size class, work order, persistence and XCD order by
-        coordinate descent over timed runs of ``steps`` steps on copies of
-        the members.  Stores and returns the chosen options."""
+        """Pick split or fused stages, tiles per size class, work order,
+        persistence and XCD order by coordinate descent over timed runs of
+        ``steps`` steps on copies of the members.  Stores and returns the
+        chosen options."""
         shadow = self._shadow()
         log = []
 
@@ -172,12 +176,16 @@ class Ensemble:
             ms = (time.perf_counter() - t0) * 1e3 / steps
             log.append((dict(o, tiles=dict(o["tiles"])), ms))
             if verbose:
-                print(f"autotune {o['order']:>10} persist={o['persist']} remap={o['remap']} "
-                      f"tiles={o['tiles']} -> {ms:.3f} ms/step", flush=True)
+                print(f"autotune split={o['split']} {o['order']:>10} persist={o['persist']} "
+                      f"remap={o['remap']} tiles={o['tiles']} -> {ms:.3f} ms/step", flush=True)
             return ms
 
         best = dict(DEFAULT_OPTIONS, tiles={})
         best_ms = timed(best)
+        trial = dict(best, split=1 - best["split"])
+        ms = timed(trial)
+        if ms < best_ms:
+            best, best_ms = trial, ms
         classes = {}
         for U in self.states:
             classes[tuple(U.n)] = classes.get(tuple(U.n), 0) + U.n[0] * U.n[1] * U.n[2]

@@ -72,6 +72,9 @@ OPTIONS = [
     {"order": "interleave", "persist": 1, "remap": 0, "graph": 0},
     {"order": "member", "persist": 2, "graph": 4,
      "tiles": {(64, 64, 1): (32, 8, 1), (16, 16, 16): (16, 16, 4), (130, 20, 1): (128, 2, 1)}},
+    {"split": 0},
+    {"split": 0, "order": "interleave", "persist": 1, "graph": 4,
+     "tiles": {(64, 64, 1): (16, 16, 1), (16, 16, 16): (16, 16, 8)}},
 ]
 
 
@@ -96,14 +99,16 @@ def test_ensemble_equals_single_runs(opt, integ):
         assert torch.equal(ens_states[i].buf, U.buf), (i, SHAPES[i])
 
 
-def test_compensated_ensemble_equals_single_runs():
+@pytest.mark.parametrize("split", [1, 0])
+def test_compensated_ensemble_equals_single_runs(split):
     from mfvm.ensemble import Ensemble
     from mfvm.numerics import time_integrators as ti
     ns, params, schemes, grids, fields, bcs = _setup(SHAPES)
     steps = 9
     ens_states = [ns.to_compensated(_state(ns, cfg, f)) for (cfg, _), f in zip(grids, fields)]
     ens = Ensemble(ens_states, [g for _, g in grids], params, schemes, bcs, ti.rk3, compensated=True)
-    dts = ens.advance(0.4, steps, {"graph": 4, "persist": 1})
+    print(f"compensated ensemble split={split}", flush=True)
+    dts = ens.advance(0.4, steps, {"graph": 4, "persist": 1, "split": split})
     torch.cuda.synchronize()
     for i, ((cfg, g), f) in enumerate(zip(grids, fields)):
         U = ns.to_compensated(_state(ns, cfg, f))
@@ -123,7 +128,7 @@ def test_autotune_keeps_results():
     ens = Ensemble(ens_states, [g for _, g in grids], params, schemes, bcs, ti.rk3)
     before = [U.buf.clone() for U in ens_states]
     best = ens.autotune(0.4, steps=2, verbose=True)
-    assert set(best) >= {"tiles", "order", "persist", "graph", "remap"}
+    assert set(best) >= {"tiles", "order", "persist", "graph", "remap", "split"}
     assert len(ens.tuning_log) >= 5
     for U, b in zip(ens_states, before):  # tuning ran on copies
         assert torch.equal(U.buf, b)
