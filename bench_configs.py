@@ -82,6 +82,20 @@ def kh128(args, torch):
     ms = _time(lambda: ns.advance(U, grid, params, schemes, bcs, _lib.INT_SSPRK3, 0.4, steps), torch)
     out = {"config": "kh128", "steps": steps, "ms": round(ms, 3),
            "Mcell_updates_per_s": round(128 * 128 * steps / (ms / 1e3) / 1e6, 2), "dtype": "f64"}
+    # The same run replayed from captured hipGraphs (50 steps per graph): the
+    # plain scheme through mfvm_closure_advance without a network, which takes
+    # the launch overhead of this small grid out of the step.
+    from mfvm import closure as cl
+    Ug = ns.allocate_u(cfg)
+    for k, f in zip(ns.VARS, fields):
+        Ug[k][1:-1, 1:-1, 1:-1] = torch.from_numpy(f).cuda()
+    Uw = ns.allocate_u(cfg)
+    Uw.buf.copy_(Ug.buf)
+    cl.advance(Uw, grid, params, schemes, bcs, None, 0.4, 50, graph=50)  # warm-up on a copy
+    msg = _time(lambda: cl.advance(Ug, grid, params, schemes, bcs, None, 0.4, steps, graph=50), torch)
+    out["graph_replay"] = {"ms": round(msg, 3),
+                           "Mcell_updates_per_s": round(128 * 128 * steps / (msg / 1e3) / 1e6, 2),
+                           "bitwise_equal_to_direct": bool(torch.equal(Ug.buf, U.buf))}
     if not args.no_cpu:
         from oracle import grid as og
         from oracle import navier_stokes as ons
