@@ -11,6 +11,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <stdlib.h>
 
 #include "../../include/mfvm.h"
 
@@ -63,7 +64,15 @@ template <int LIM, class R> MFVM_DEV R limit(R b, R f) {
     return same ? copysign(least, b) : (opp ? third : zero);
   } else {
     // slope_limiters.py:94-103 (harmonic mean on denom > 0, else 0)
-    const R harm = qdiv(((R(2) * b) * f), (fabs(b) + fabs(f)));
+    R den = fabs(b) + fabs(f);
+    // float32: the hardware reciprocal of a denominator below 2^-126 is inf
+    // (and (2 b f) underflows to 0, so the product is NaN).  Fields that start
+    // at exactly zero (Maxwell's B) pass through that range.  The harmonic
+    // mean is below half the denominator, so with the denominator raised to
+    // 2^-100 both it and the true value are below 2^-101.  (A NaN slope still
+    // yields NaN: `same` is false and `zero` carries it.)
+    if constexpr (sizeof(R) == sizeof(float)) den = fmaxf(den, 0x1p-100f);
+    const R harm = qdiv(((R(2) * b) * f), den);
     return same ? copysign(harm, b) : zero;
   }
 }
@@ -136,6 +145,16 @@ MFVM_DEV TileId tile_id(bool remap) {
   t.y = (L / gx) % gy;
   t.z = L / (gx * gy);
   return t;
+}
+
+// XCD-aware tile order (tile_id) on the host side; MFVM_XCD_REMAP=0 selects
+// the plain order.
+inline int xcd_remap_enabled() {
+  static const int on = [] {
+    const char* e = getenv("MFVM_XCD_REMAP");
+    return e ? atoi(e) : 1;
+  }();
+  return on;
 }
 
 // Ghosted SoA addressing: field-major, x1 fastest.

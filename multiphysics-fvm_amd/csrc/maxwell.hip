@@ -122,6 +122,25 @@ MFVM_DEV void mx_face(const MXPar& p, const R* l, const R* r, R area, R* fs) {
   }
 }
 
+// MUSCL states of one cell along D from its stencil (qm, q, qp): limited
+// slope s = limiter(back, front) * L with back/front = difference / L, then
+// hi = q + s/2 (upper face), lo = q - s/2 (reconstructors.py:115-148).
+template <int D, int LIM, class R>
+MFVM_DEV void mx_slopes(const Geo& geo, int i, int j, int k, const R* qm, const R* q, const R* qp, R lo[6],
+                        R hi[6]) {
+  const R len = geo.lpow2[D] ? R(1.0 / geo.linv[D]) : met<R>(geo, MFVM_MET_L1 + D, i - 1, j - 1, k - 1);
+#pragma unroll
+  for (int f = 0; f < 6; ++f) {
+    const R b = div_len<R>(geo, D, q[f] - qm[f], len);
+    const R w = div_len<R>(geo, D, qp[f] - q[f], len);
+    const R s = limit<LIM, R>(b, w) * len;
+    hi[f] = q[f] + s / R(2);
+    lo[f] = q[f] - s / R(2);
+  }
+}
+
+// States of ghosted cell (i,j,k) along D, loading its stencil; ghost cells
+// carry zero slope (reconstructors.py:127-132).
 template <int D, int LIM, class R>
 MFVM_DEV void mx_states(const MXArgs& a, const R* U, int i, int j, int k, R lo[6], R hi[6]) {
   const long long fs = a.lay.fs;
@@ -137,55 +156,22 @@ MFVM_DEV void mx_states(const MXArgs& a, const R* U, int i, int j, int k, R lo[6
     for (int f = 0; f < 6; ++f) lo[f] = hi[f] = q[f];
     return;
   }
-  const R len = a.geo.lpow2[D] ? R(1.0 / a.geo.linv[D]) : met<R>(a.geo, MFVM_MET_L1 + D, i - 1, j - 1, k - 1);
+  R qm[6], qp[6];
 #pragma unroll
   for (int f = 0; f < 6; ++f) {
-    const R qm = U[o + f * fs - st], qp = U[o + f * fs + st];
-    const R b = div_len<R>(a.geo, D, q[f] - qm, len);
-    const R w = div_len<R>(a.geo, D, qp - q[f], len);
-    const R s = limit<LIM, R>(b, w) * len;
-    hi[f] = q[f] + s / R(2);
-    lo[f] = q[f] - s / R(2);
+    qm[f] = U[o + f * fs - st];
+    qp[f] = U[o + f * fs + st];
   }
+  mx_slopes<D, LIM, R>(a.geo, i, j, k, qm, q, qp, lo, hi);
 }
 
-template <int D, int LIM, int RS, class R>
-MFVM_DEV void mx_dir(const MXArgs& a, const R* U, int i, int j, int k, R* acc) {
-  const int di = D == 0, dj = D == 1, dk = D == 2;
-  R lo_m[6], hi_m[6], lo_c[6], hi_c[6], lo_p[6], hi_p[6];
-  mx_states<D, LIM, R>(a, U, i - di, j - dj, k - dk, lo_m, hi_m);
-  mx_states<D, LIM, R>(a, U, i, j, k, lo_c, hi_c);
-  mx_states<D, LIM, R>(a, U, i + di, j + dj, k + dk, lo_p, hi_p);
-  const int m = MFVM_MET_S1 + D;
-  const R s_lo = met<R>(a.geo, m, i - 1, j - 1, k - 1);
-  const R s_hi = met<R>(a.geo, m, i - 1 + di, j - 1 + dj, k - 1 + dk);
-  R f_lo[6], f_hi[6];
-  mx_face<D, RS, R>(a.p, hi_m, lo_c, s_lo, f_lo);
-  mx_face<D, RS, R>(a.p, hi_c, lo_p, s_hi, f_hi);
-#pragma unroll
-  for (int q = 0; q < 6; ++q) {
-    if (D == 0) acc[q] = f_hi[q] - f_lo[q];
-    else acc[q] = (acc[q] + f_hi[q]) - f_lo[q];
-  }
-}
-
-template <int LIM, int RS, class R>
-__global__ __launch_bounds__(256) void maxwell_stage_kernel(MXArgs a) {
-  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = 1 + blockIdx.z;
-  if (i > a.lay.n1 || j > a.lay.n2) return;
-  const R* U = static_cast<const R*>(a.U);
-  const R* Rr = static_cast<const R*>(a.Rr);
-  R acc[6];
-  mx_dir<0, LIM, RS, R>(a, U, i, j, k, acc);
-  mx_dir<1, LIM, RS, R>(a, U, i, j, k, acc);
-  mx_dir<2, LIM, RS, R>(a, U, i, j, k, acc);
-
+// Right-hand side and stage update of interior cell (i,j,k) from its flux
+// differences acc = ((((F1+ - F1-) + F2+) - F2-) + F3+) - F3- and its values
+// u: rhs = -acc/V + ((sg + sp) + sc) (maxwell.py:708, 722-727).
+template <class R>
+MFVM_DEV void mx_finish(const MXArgs& a, const R* __restrict__ Rr, int i, int j, int k, const R acc[6],
+                        const R u[6]) {
   const long long o = a.lay.at(i, j, k), fs = a.lay.fs;
-  R u[6];
-#pragma unroll
-  for (int q = 0; q < 6; ++q) u[q] = U[o + q * fs];
   // geometric source (exactly +-0 on Cartesian grids)
   R sg[6] = {R(0), R(0), R(0), R(0), R(0), R(0)};
   if (a.geo.curv) {
@@ -236,6 +222,256 @@ __global__ __launch_bounds__(256) void maxwell_stage_kernel(MXArgs a) {
   stage_store<R>(a.io, 6, fs, a.il.fs, o, a.il.at(i, j, k), u, kr, nullptr);
 }
 
+// Tile of the stage kernel: 64 x 4 (x1, x2) columns, one thread each.
+constexpr int MX_BX = 64, MX_BY = 4;
+
+// LDS of the in-plane exchange: RX[6][BY][BX+1] lower-face x1 states (the
+// last column: the cell beyond the tile), LX[6][BY] upper-face x1 states of
+// the cells before the tile, RY[6][BY+1][BX] / LY[6][BX] likewise for x2.
+// The face fluxes reuse the RX / RY space once every state is read.
+constexpr int MX_NRX = MX_BY * (MX_BX + 1), MX_NRY = (MX_BY + 1) * MX_BX;
+constexpr size_t mx_lds_elems() { return 6 * (MX_NRX + MX_BY + MX_NRY + MX_BX); }
+
+// In-plane flux differences of plane m of the tile:
+// acc = ((F1(i+1/2) - F1(i-1/2)) + F2(j+1/2)) - F2(j-1/2).  Every thread of
+// the workgroup calls it (three barriers); cu: the own cell's values.
+template <int LIM, int RS, class R>
+MFVM_DEV void mx_inplane(const MXArgs& a, const R* __restrict__ U, R* sm, int i0, int j0, int m, const R cu[6],
+                         R acc[6]) {
+  constexpr int BX = MX_BX, BY = MX_BY, NT = MX_BX * MX_BY, wx = MX_BX + 1;
+  constexpr int nrx = MX_NRX, nry = MX_NRY;
+  R* RX = sm;
+  R* LX = RX + 6 * nrx;
+  R* RY = LX + 6 * BY;
+  R* LY = RY + 6 * nry;
+  R* FX = RX;
+  R* FY = RY;
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * BX + tx;
+  const int n1 = a.lay.n1, n2 = a.lay.n2;
+  const long long fs = a.lay.fs;
+  const int i = i0 + tx, j = j0 + ty;
+  const bool own = i <= n1 && j <= n2;
+  __syncthreads();  // the previous plane's flux reads are done
+  R hx[6], hy[6];
+  if (j <= n2 && i <= n1 + 1) {
+    R lo[6];
+    if (i <= n1) {
+      if (LIM == MFVM_LIM_NONE) {
+#pragma unroll
+        for (int f = 0; f < 6; ++f) lo[f] = hx[f] = cu[f];
+      } else {
+        const long long o = a.lay.at(i, j, m);
+        R qm[6], qp[6];
+#pragma unroll
+        for (int f = 0; f < 6; ++f) {
+          qm[f] = U[o + f * fs - 1];
+          qp[f] = U[o + f * fs + 1];
+        }
+        mx_slopes<0, LIM, R>(a.geo, i, j, m, qm, cu, qp, lo, hx);
+      }
+    } else {
+      mx_states<0, LIM, R>(a, U, i, j, m, lo, hx);  // x1 ghost: raw
+    }
+#pragma unroll
+    for (int f = 0; f < 6; ++f) RX[f * nrx + ty * wx + tx] = lo[f];
+  }
+  if (i <= n1 && j <= n2 + 1) {
+    R lo[6];
+    if (j <= n2) {
+      if (LIM == MFVM_LIM_NONE) {
+#pragma unroll
+        for (int f = 0; f < 6; ++f) lo[f] = hy[f] = cu[f];
+      } else {
+        const long long o = a.lay.at(i, j, m);
+        R qm[6], qp[6];
+#pragma unroll
+        for (int f = 0; f < 6; ++f) {
+          qm[f] = U[o + f * fs - a.lay.sy];
+          qp[f] = U[o + f * fs + a.lay.sy];
+        }
+        mx_slopes<1, LIM, R>(a.geo, i, j, m, qm, cu, qp, lo, hy);
+      }
+    } else {
+      mx_states<1, LIM, R>(a, U, i, j, m, lo, hy);  // x2 ghost: raw
+    }
+#pragma unroll
+    for (int f = 0; f < 6; ++f) RY[f * nry + ty * BX + tx] = lo[f];
+  }
+  // helpers: [0, 2BX) x2 tile-edge cells, [2BX, 2BX+2BY) x1 tile-edge cells
+  if (tid < 2 * (BX + BY)) {
+    const int t = tid;
+    R lo[6], hi[6];
+    if (t < 2 * BX) {
+      const int c = t % BX, side = t / BX, ii = i0 + c;
+      if (ii <= n1) {
+        if (side == 0) {
+          mx_states<1, LIM, R>(a, U, ii, j0 - 1, m, lo, hi);
+#pragma unroll
+          for (int f = 0; f < 6; ++f) LY[f * BX + c] = hi[f];
+        } else if (j0 + BY <= n2 + 1) {
+          mx_states<1, LIM, R>(a, U, ii, j0 + BY, m, lo, hi);
+#pragma unroll
+          for (int f = 0; f < 6; ++f) RY[f * nry + BY * BX + c] = lo[f];
+        }
+      }
+    } else {
+      const int u = t - 2 * BX, r = u % BY, side = u / BY, jj = j0 + r;
+      if (jj <= n2) {
+        if (side == 0) {
+          mx_states<0, LIM, R>(a, U, i0 - 1, jj, m, lo, hi);
+#pragma unroll
+          for (int f = 0; f < 6; ++f) LX[f * BY + r] = hi[f];
+        } else if (i0 + BX <= n1 + 1) {
+          mx_states<0, LIM, R>(a, U, i0 + BX, jj, m, lo, hi);
+#pragma unroll
+          for (int f = 0; f < 6; ++f) RX[f * nrx + r * wx + BX] = lo[f];
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // fluxes: the own cell's upper x1 and x2 faces; the tile's lower-edge
+  // faces by the last BX + BY threads
+  R fx[6], fy[6], fh[6];
+  if (own) {
+    R rr[6];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) rr[f] = RX[f * nrx + ty * wx + tx + 1];
+    mx_face<0, RS, R>(a.p, hx, rr, met<R>(a.geo, MFVM_MET_S1, i, j - 1, m - 1), fx);
+#pragma unroll
+    for (int f = 0; f < 6; ++f) rr[f] = RY[f * nry + (ty + 1) * BX + tx];
+    mx_face<1, RS, R>(a.p, hy, rr, met<R>(a.geo, MFVM_MET_S2, i - 1, j, m - 1), fy);
+  }
+  const int u = NT - 1 - tid;
+  int hkind = 0, hidx = 0;
+  if (u < BX + BY) {
+    R ll[6], rr[6];
+    if (u < BX) {
+      const int c = u, ii = i0 + c;
+      if (ii <= n1) {
+#pragma unroll
+        for (int f = 0; f < 6; ++f) {
+          ll[f] = LY[f * BX + c];
+          rr[f] = RY[f * nry + c];
+        }
+        mx_face<1, RS, R>(a.p, ll, rr, met<R>(a.geo, MFVM_MET_S2, ii - 1, j0 - 1, m - 1), fh);
+        hkind = 1;
+        hidx = c;
+      }
+    } else {
+      const int r = u - BX, jj = j0 + r;
+      if (jj <= n2) {
+#pragma unroll
+        for (int f = 0; f < 6; ++f) {
+          ll[f] = LX[f * BY + r];
+          rr[f] = RX[f * nrx + r * wx];
+        }
+        mx_face<0, RS, R>(a.p, ll, rr, met<R>(a.geo, MFVM_MET_S1, i0 - 1, jj - 1, m - 1), fh);
+        hkind = 2;
+        hidx = r * wx;
+      }
+    }
+  }
+  __syncthreads();  // every state is read: the fluxes may overwrite them
+  if (own) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      FX[q * nrx + ty * wx + tx + 1] = fx[q];
+      FY[q * nry + (ty + 1) * BX + tx] = fy[q];
+    }
+  }
+  if (hkind == 1) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) FY[q * nry + hidx] = fh[q];
+  } else if (hkind == 2) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) FX[q * nrx + hidx] = fh[q];
+  }
+  __syncthreads();
+  if (own) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const R ax = FX[q * nrx + ty * wx + tx + 1] - FX[q * nrx + ty * wx + tx];
+      acc[q] = (ax + FY[q * nry + (ty + 1) * BX + tx]) - FY[q * nry + ty * BX + tx];
+    }
+  }
+}
+
+// Stage kernel: one workgroup per 64 x 4 tile and chunk of kz planes of x3,
+// one thread per (x1, x2) column.  The x3 states come from a register window
+// over planes m-1, m, m+1 of the column; the x1/x2 states of plane m are
+// exchanged through LDS (mx_inplane).  Every face flux is computed once,
+// with the per-face arithmetic and the summation order of maxwell.py:680-727.
+template <int LIM, int RS, class R>
+__global__ __launch_bounds__(MX_BX * MX_BY) void maxwell_stage_kernel(MXArgs a, int kz, int remap) {
+  extern __shared__ double smem_raw[];
+  R* sm = reinterpret_cast<R*>(smem_raw);
+  const TileId tl = tile_id(remap != 0);
+  const int n1 = a.lay.n1, n2 = a.lay.n2, n3 = a.lay.n3;
+  const long long fs = a.lay.fs;
+  const int i0 = 1 + tl.x * MX_BX, j0 = 1 + tl.y * MX_BY;
+  const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
+  const int kb = 1 + tl.z * kz, ke = min(kb + kz, n3 + 1);
+  const bool own = i <= n1 && j <= n2;
+  const R* __restrict__ U = static_cast<const R*>(a.U);
+  const R* __restrict__ Rr = static_cast<const R*>(a.Rr);
+  auto load6 = [&](int k, R* w) {
+    const long long o = a.lay.at(i, j, k);
+#pragma unroll
+    for (int f = 0; f < 6; ++f) w[f] = U[o + f * fs];
+  };
+
+  // window of the own column: zp (plane m-1), cu (m), nx (m+1); lz_prev: the
+  // upper x3 state of cell m-1; fz_prev: F.S of face m-3/2
+  R zp[6], cu[6], nx[6], lz_prev[6], fz_prev[6], acc_prev[6];
+  if (own) {
+    load6(kb - 1, zp);
+    load6(kb, cu);
+    if (kb - 1 == 0 || LIM == MFVM_LIM_NONE) {
+#pragma unroll
+      for (int f = 0; f < 6; ++f) lz_prev[f] = zp[f];
+    } else {
+      R zm[6], lo[6];
+      load6(kb - 2, zm);
+      mx_slopes<2, LIM, R>(a.geo, i, j, kb - 1, zm, zp, cu, lo, lz_prev);
+    }
+  }
+  for (int m = kb; m <= ke; ++m) {
+    // x3 face m-1/2: left = upper state of cell m-1, right = lower state of cell m
+    R fz[6], lz_cur[6];
+    if (own) {
+      if (m + 1 <= n3 + 1) load6(m + 1, nx);
+      R lo[6];
+      if (m == n3 + 1 || LIM == MFVM_LIM_NONE) {
+#pragma unroll
+        for (int f = 0; f < 6; ++f) lo[f] = lz_cur[f] = cu[f];
+      } else {
+        mx_slopes<2, LIM, R>(a.geo, i, j, m, zp, cu, nx, lo, lz_cur);
+      }
+      mx_face<2, RS, R>(a.p, lz_prev, lo, met<R>(a.geo, MFVM_MET_S3, i - 1, j - 1, m - 1), fz);
+    }
+    R accm[6];
+    if (m < ke) mx_inplane<LIM, RS, R>(a, U, sm, i0, j0, m, cu, accm);
+    // cell m-1: its x3 difference completes the divergence
+    if (own && m > kb) {
+      R acc[6];
+#pragma unroll
+      for (int q = 0; q < 6; ++q) acc[q] = (acc_prev[q] + fz[q]) - fz_prev[q];
+      mx_finish<R>(a, Rr, i, j, m - 1, acc, zp);
+    }
+    if (own) {
+#pragma unroll
+      for (int f = 0; f < 6; ++f) {
+        fz_prev[f] = fz[f];
+        lz_prev[f] = lz_cur[f];
+        acc_prev[f] = accm[f];
+        zp[f] = cu[f];
+        cu[f] = nx[f];
+      }
+    }
+  }
+}
+
 // dt = cfl / (max_d c/Lmin_d + (2 chi_max) smax)   (maxwell.py:579-598)
 __global__ void maxwell_dt_kernel(double c, double l1, double l2, double l3, double chimax, double smax,
                                   double cfl, double* dt_out) {
@@ -275,12 +511,17 @@ hipError_t maxwell_launch_res(const mfvm_grid* g, const mfvm_maxwell_params* q, 
   return hipGetLastError();
 }
 
+// x3 planes per workgroup of the stage kernel.
+constexpr int MX_KZ = 16;
+
 template <int LIM, int RS, class R>
 static hipError_t mx_launch_t(const MXArgs& a, hipStream_t s) {
-  const dim3 blk(64, 4, 1);
-  const dim3 grd((a.lay.n1 + 63) / 64, (a.lay.n2 + 3) / 4, a.lay.n3);
+  const int kz = a.lay.n3 < MX_KZ ? a.lay.n3 : MX_KZ;
+  const dim3 blk(MX_BX, MX_BY, 1);
+  const dim3 grd((a.lay.n1 + MX_BX - 1) / MX_BX, (a.lay.n2 + MX_BY - 1) / MX_BY, (a.lay.n3 + kz - 1) / kz);
+  const size_t lds = mx_lds_elems() * sizeof(R);
   const int tk = t_begin(MFVM_TK_STAGE, s);
-  hipLaunchKernelGGL((maxwell_stage_kernel<LIM, RS, R>), grd, blk, 0, s, a);
+  hipLaunchKernelGGL((maxwell_stage_kernel<LIM, RS, R>), grd, blk, lds, s, a, kz, xcd_remap_enabled());
   t_end(tk, s);
   return hipGetLastError();
 }

@@ -1146,13 +1146,4 @@ inline bool buf_src_ok(const Layout& lay, size_t rsize) {
   return (unsigned long long)NS_NC * (unsigned long long)lay.fs * rsize < (1ull << 32);
 }
 
-// XCD-aware tile order (tile_id); MFVM_XCD_REMAP=0 selects the plain order.
-inline int xcd_remap_enabled() {
-  static const int on = [] {
-    const char* e = getenv("MFVM_XCD_REMAP");
-    return e ? atoi(e) : 1;
-  }();
-  return on;
-}
-
 }  // namespace mfvm
