@@ -190,8 +190,9 @@ typedef struct mfvm_ns_work {   /* scratch for whole steps (all physics) */
   void* kacc;                   /* interior, 5 fields */
   double* partials;             /* >= mfvm_ns_partials_size() doubles */
   double* dts;                  /* device: dt of step s written to dts[s] */
-  void* xy;                     /* optional (NS, f64): interior 5-field scratch for split
-                                   stages (see mfvm_stage.xy); NULL: fused stages */
+  void* xy;                     /* optional, interior scratch for split stages in the state
+                                   dtype: NS f64 5 fields (see mfvm_stage.xy), Maxwell 6
+                                   fields; NULL: fused stages */
 } mfvm_ns_work;
 
 /* ---- library ------------------------------------------------------------------ */

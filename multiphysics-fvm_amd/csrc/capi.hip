@@ -17,7 +17,7 @@ template <class R> hipError_t launch_copy_interior(const mfvm_grid*, int, const 
 template <class R> hipError_t heat_launch_stage(const mfvm_grid*, const mfvm_heat_params*, const void*, const StageIO&, hipStream_t);
 hipError_t heat_launch_dt(const mfvm_grid*, const mfvm_heat_params*, double, double*, hipStream_t);
 template <class R> hipError_t maxwell_launch_res(const mfvm_grid*, const mfvm_maxwell_params*, const void*, void*, hipStream_t);
-template <class R> hipError_t maxwell_launch_stage(const mfvm_grid*, const mfvm_maxwell_params*, const void*, const void*, const StageIO&, hipStream_t);
+template <class R> hipError_t maxwell_launch_stage(const mfvm_grid*, const mfvm_maxwell_params*, const void*, const void*, void*, const StageIO&, hipStream_t);
 hipError_t maxwell_launch_dt(const mfvm_grid*, const mfvm_maxwell_params*, double, double*, hipStream_t);
 template <class R> hipError_t launch_cfl_partials(const mfvm_grid*, const mfvm_ns_params*, const void*, const void*, double*, hipStream_t);
 hipError_t launch_dt_append(const mfvm_grid*, const mfvm_ns_params*, const double*, double, double, double*, double*,
@@ -830,6 +830,7 @@ static int check_mx(const mfvm_grid* g, const mfvm_maxwell_params* p) {
 struct MXCtx {
   const mfvm_grid* g;
   const mfvm_maxwell_params* p;
+  void* xy;  // split stage scratch (mfvm_ns_work.xy), NULL: fused stage
 };
 static int mx_pre(void* c, const void* in, void* R, void* stream) {
   MXCtx* x = static_cast<MXCtx*>(c);
@@ -838,8 +839,8 @@ static int mx_pre(void* c, const void* in, void* R, void* stream) {
 static int mx_stage_op(void* c, const void* in, const void* R, const StageIO& io, double*, void* stream) {
   MXCtx* x = static_cast<MXCtx*>(c);
   hipStream_t s = static_cast<hipStream_t>(stream);
-  return hip_check(x->p->dtype == MFVM_F64 ? maxwell_launch_stage<double>(x->g, x->p, in, R, io, s)
-                                           : maxwell_launch_stage<float>(x->g, x->p, in, R, io, s),
+  return hip_check(x->p->dtype == MFVM_F64 ? maxwell_launch_stage<double>(x->g, x->p, in, R, x->xy, io, s)
+                                           : maxwell_launch_stage<float>(x->g, x->p, in, R, x->xy, io, s),
                    "maxwell_stage");
 }
 
@@ -859,7 +860,7 @@ int mfvm_maxwell_rhs(const mfvm_grid* g, const mfvm_maxwell_params* p, const voi
   int rc = mfvm_maxwell_residual(g, p, U, R, stream);
   if (rc) return rc;
   if (!rhs) return fail(MFVM_E_ARG, "maxwell_rhs: NULL output");
-  MXCtx x{g, p};
+  MXCtx x{g, p, nullptr};
   StageIO io = {};
   io.kacc = rhs;
   io.update = MFVM_UPD_NONE;
@@ -879,7 +880,7 @@ int mfvm_maxwell_step(const mfvm_grid* g, const mfvm_maxwell_params* p, const mf
   int rc = check_mx(g, p);
   if (!rc) rc = check_work(w, integrator, true);
   if (rc) return rc;
-  MXCtx x{g, p};
+  MXCtx x{g, p, w->xy};
   PhysOps ph{6, p->dtype, &p->bc[0][0], &x, mx_pre, mx_stage_op};
   return step_impl(g, ph, w, integrator, nullptr, dt, nullptr, true, stream);
 }
@@ -890,7 +891,7 @@ int mfvm_maxwell_advance(const mfvm_grid* g, const mfvm_maxwell_params* p, const
   if (!rc) rc = check_work(w, integrator, true);
   if (rc) return rc;
   if (!w->dts || nsteps < 0) return fail(MFVM_E_ARG, "maxwell_advance: bad dts / nsteps");
-  MXCtx x{g, p};
+  MXCtx x{g, p, w->xy};
   PhysOps ph{6, p->dtype, &p->bc[0][0], &x, mx_pre, mx_stage_op};
   for (int n = 0; n < nsteps; ++n) {
     // Maxwell dt depends on the grid and parameters only (maxwell.py:579-598)

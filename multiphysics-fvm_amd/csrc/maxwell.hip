@@ -28,6 +28,7 @@ struct MXArgs {
   const void* U;
   const void* Rr;  // residuals [2][ghosted]
   StageIO io;
+  void* xy;        // split stage: interior [6] in-plane flux differences
 };
 
 template <class R>
@@ -397,13 +398,15 @@ MFVM_DEV void mx_inplane(const MXArgs& a, const R* __restrict__ U, R* sm, int i0
   }
 }
 
-// Stage kernel: one workgroup per 64 x 4 tile and chunk of kz planes of x3,
+// Stage body: one workgroup per 64 x 4 tile and chunk of kz planes of x3,
 // one thread per (x1, x2) column.  The x3 states come from a register window
 // over planes m-1, m, m+1 of the column; the x1/x2 states of plane m are
-// exchanged through LDS (mx_inplane).  Every face flux is computed once,
-// with the per-face arithmetic and the summation order of maxwell.py:680-727.
-template <int LIM, int RS, class R>
-__global__ __launch_bounds__(MX_BX * MX_BY) void maxwell_stage_kernel(MXArgs a, int kz, int remap) {
+// exchanged through LDS (mx_inplane), or, ZONLY, their flux differences are
+// read from a.xy (written by maxwell_xy_kernel).  Every face flux is
+// computed once, with the per-face arithmetic and the summation order of
+// maxwell.py:680-727.
+template <int LIM, int RS, class R, bool ZONLY>
+MFVM_DEV void mx_stage_body(const MXArgs& a, int kz, int remap) {
   extern __shared__ double smem_raw[];
   R* sm = reinterpret_cast<R*>(smem_raw);
   const TileId tl = tile_id(remap != 0);
@@ -451,7 +454,18 @@ __global__ __launch_bounds__(MX_BX * MX_BY) void maxwell_stage_kernel(MXArgs a, 
       mx_face<2, RS, R>(a.p, lz_prev, lo, met<R>(a.geo, MFVM_MET_S3, i - 1, j - 1, m - 1), fz);
     }
     R accm[6];
-    if (m < ke) mx_inplane<LIM, RS, R>(a, U, sm, i0, j0, m, cu, accm);
+    if (m < ke) {
+      if constexpr (ZONLY) {
+        if (own) {
+          const R* xy = static_cast<const R*>(a.xy);
+          const long long oi = a.il.at(i, j, m);
+#pragma unroll
+          for (int q = 0; q < 6; ++q) accm[q] = xy[oi + q * a.il.fs];
+        }
+      } else {
+        mx_inplane<LIM, RS, R>(a, U, sm, i0, j0, m, cu, accm);
+      }
+    }
     // cell m-1: its x3 difference completes the divergence
     if (own && m > kb) {
       R acc[6];
@@ -470,6 +484,50 @@ __global__ __launch_bounds__(MX_BX * MX_BY) void maxwell_stage_kernel(MXArgs a, 
       }
     }
   }
+}
+
+template <int LIM, int RS, class R>
+__global__ __launch_bounds__(MX_BX * MX_BY) void maxwell_stage_kernel(MXArgs a, int kz, int remap) {
+  mx_stage_body<LIM, RS, R, false>(a, kz, remap);
+}
+
+// Split stage (float64).  The fused float64 body holds the x3 window, the
+// x1/x2 exchange and the hoisted metric factors at once: over 256 VGPRs,
+// one wave per SIMD.  Each split pass fits two waves per SIMD with a small
+// spill, which measures 1.2x faster at 128x128x256.  (The float32 fused body
+// already runs two waves and is faster than the split.)  x1/x2 pass: the
+// in-plane flux differences of every cell of the tile's planes, into a.xy.
+template <int LIM, int RS, class R>
+__global__ __launch_bounds__(MX_BX * MX_BY, 2) void maxwell_xy_kernel(MXArgs a, int kz, int remap) {
+  extern __shared__ double smem_raw[];
+  R* sm = reinterpret_cast<R*>(smem_raw);
+  const TileId tl = tile_id(remap != 0);
+  const int i0 = 1 + tl.x * MX_BX, j0 = 1 + tl.y * MX_BY;
+  const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
+  const int kb = 1 + tl.z * kz, ke = min(kb + kz, a.lay.n3 + 1);
+  const bool own = i <= a.lay.n1 && j <= a.lay.n2;
+  const R* __restrict__ U = static_cast<const R*>(a.U);
+  R* __restrict__ xy = static_cast<R*>(a.xy);
+  for (int m = kb; m < ke; ++m) {
+    R cu[6], acc[6];
+    if (own) {
+      const long long o = a.lay.at(i, j, m);
+#pragma unroll
+      for (int f = 0; f < 6; ++f) cu[f] = U[o + f * a.lay.fs];
+    }
+    mx_inplane<LIM, RS, R>(a, U, sm, i0, j0, m, cu, acc);
+    if (own) {
+      const long long oi = a.il.at(i, j, m);
+#pragma unroll
+      for (int q = 0; q < 6; ++q) xy[oi + q * a.il.fs] = acc[q];
+    }
+  }
+}
+
+// x3 pass: x3 faces, sources and the stage update.
+template <int LIM, int RS, class R>
+__global__ __launch_bounds__(MX_BX * MX_BY, 2) void maxwell_z_kernel(MXArgs a, int kz, int remap) {
+  mx_stage_body<LIM, RS, R, true>(a, kz, remap);
 }
 
 // dt = cfl / (max_d c/Lmin_d + (2 chi_max) smax)   (maxwell.py:579-598)
@@ -511,8 +569,10 @@ hipError_t maxwell_launch_res(const mfvm_grid* g, const mfvm_maxwell_params* q, 
   return hipGetLastError();
 }
 
-// x3 planes per workgroup of the stage kernel.
-constexpr int MX_KZ = 16;
+// x3 planes per workgroup: the fused stage and the x3 pass of the split
+// stage (MX_KZ), the x1/x2 pass (MX_XY_KZ: it keeps nothing from one plane
+// to the next, so short chunks only add workgroups).
+constexpr int MX_KZ = 16, MX_XY_KZ = 8;
 
 template <int LIM, int RS, class R>
 static hipError_t mx_launch_t(const MXArgs& a, hipStream_t s) {
@@ -520,16 +580,29 @@ static hipError_t mx_launch_t(const MXArgs& a, hipStream_t s) {
   const dim3 blk(MX_BX, MX_BY, 1);
   const dim3 grd((a.lay.n1 + MX_BX - 1) / MX_BX, (a.lay.n2 + MX_BY - 1) / MX_BY, (a.lay.n3 + kz - 1) / kz);
   const size_t lds = mx_lds_elems() * sizeof(R);
+  const int remap = xcd_remap_enabled();
+  if (a.xy && sizeof(R) == sizeof(double)) {
+    const int kx = a.lay.n3 < MX_XY_KZ ? a.lay.n3 : MX_XY_KZ;
+    const dim3 gx(grd.x, grd.y, (a.lay.n3 + kx - 1) / kx);
+    const int tx = t_begin(MFVM_TK_STAGE, s);
+    hipLaunchKernelGGL((maxwell_xy_kernel<LIM, RS, R>), gx, blk, lds, s, a, kx, remap);
+    t_end(tx, s);
+    const int tz = t_begin(MFVM_TK_OTHER, s);
+    hipLaunchKernelGGL((maxwell_z_kernel<LIM, RS, R>), grd, blk, 0, s, a, kz, remap);
+    t_end(tz, s);
+    return hipGetLastError();
+  }
   const int tk = t_begin(MFVM_TK_STAGE, s);
-  hipLaunchKernelGGL((maxwell_stage_kernel<LIM, RS, R>), grd, blk, lds, s, a, kz, xcd_remap_enabled());
+  hipLaunchKernelGGL((maxwell_stage_kernel<LIM, RS, R>), grd, blk, lds, s, a, kz, remap);
   t_end(tk, s);
   return hipGetLastError();
 }
 
 template <class R>
 hipError_t maxwell_launch_stage(const mfvm_grid* g, const mfvm_maxwell_params* q, const void* U,
-                                const void* Rr, const StageIO& io, hipStream_t s) {
+                                const void* Rr, void* xy, const StageIO& io, hipStream_t s) {
   MXArgs a;
+  a.xy = xy;
   a.lay = make_layout(g->n[0], g->n[1], g->n[2]);
   a.il = make_ilayout(g->n[0], g->n[1], g->n[2]);
   a.geo = make_geo(g);
@@ -559,7 +632,7 @@ hipError_t maxwell_launch_dt(const mfvm_grid* g, const mfvm_maxwell_params* q, d
 
 template hipError_t maxwell_launch_res<double>(const mfvm_grid*, const mfvm_maxwell_params*, const void*, void*, hipStream_t);
 template hipError_t maxwell_launch_res<float>(const mfvm_grid*, const mfvm_maxwell_params*, const void*, void*, hipStream_t);
-template hipError_t maxwell_launch_stage<double>(const mfvm_grid*, const mfvm_maxwell_params*, const void*, const void*, const StageIO&, hipStream_t);
-template hipError_t maxwell_launch_stage<float>(const mfvm_grid*, const mfvm_maxwell_params*, const void*, const void*, const StageIO&, hipStream_t);
+template hipError_t maxwell_launch_stage<double>(const mfvm_grid*, const mfvm_maxwell_params*, const void*, const void*, void*, const StageIO&, hipStream_t);
+template hipError_t maxwell_launch_stage<float>(const mfvm_grid*, const mfvm_maxwell_params*, const void*, const void*, void*, const StageIO&, hipStream_t);
 
 }  // namespace mfvm

@@ -1,6 +1,7 @@
 """Maxwell in B/D form with cleaning (drop-in for src/physics/maxwell.py)."""
 
 import ctypes
+import os
 
 import torch
 
@@ -74,6 +75,10 @@ def work(U):
     w.S2 = U.scratch("S2", tuple(U.buf.shape)).data_ptr()
     w.C = _res_buf(U).data_ptr()
     w.kacc = U.scratch("kacc", (6, n3, n2, n1)).data_ptr()
+    # split stages (x1/x2 flux differences, then the x3 pass; identical
+    # results); MFVM_SPLIT=0 selects the fused stage kernel
+    if os.environ.get("MFVM_SPLIT", "1") != "0":
+        w.xy = U.scratch("xy", (6, n3, n2, n1)).data_ptr()
     return w
 
 
