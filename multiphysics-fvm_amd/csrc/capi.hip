@@ -16,6 +16,7 @@ template <class R> hipError_t launch_bcs(const mfvm_grid*, int, const int32_t*, 
 template <class R> hipError_t launch_copy_interior(const mfvm_grid*, int, const void*, void*, hipStream_t);
 template <class R> hipError_t heat_launch_stage(const mfvm_grid*, const mfvm_heat_params*, const void*, const StageIO&, hipStream_t);
 hipError_t heat_launch_dt(const mfvm_grid*, const mfvm_heat_params*, double, double*, hipStream_t);
+hipError_t heat_launch_dt_fill(double*, int, hipStream_t);
 template <class R> hipError_t maxwell_launch_res(const mfvm_grid*, const mfvm_maxwell_params*, const void*, void*, hipStream_t);
 template <class R> hipError_t maxwell_launch_stage(const mfvm_grid*, const mfvm_maxwell_params*, const void*, const void*, void*, const StageIO&, hipStream_t);
 hipError_t maxwell_launch_dt(const mfvm_grid*, const mfvm_maxwell_params*, double, double*, hipStream_t);
@@ -805,12 +806,42 @@ int mfvm_heat_advance(const mfvm_grid* g, const mfvm_heat_params* p, const mfvm_
   if (!w->dts || nsteps < 0) return fail(MFVM_E_ARG, "heat_advance: bad dts / nsteps");
   HeatCtx x{g, p};
   PhysOps ph{1, p->dtype, &p->bc[0][0], &x, nullptr, heat_stage_op};
-  for (int n = 0; n < nsteps; ++n) {
-    // heat dt depends on the grid only (heat.py:338-349)
-    if ((rc = mfvm_heat_dt(g, p, cfl, w->dts + n, stream))) return rc;
-    if ((rc = step_impl(g, ph, w, integrator, w->dts + n, 0.0, nullptr, n == nsteps - 1, stream))) return rc;
+  // heat dt depends on the parameters and the grid only (heat.py:338-349):
+  // computed once, every step's entry holds that value
+  if ((rc = mfvm_heat_dt(g, p, cfl, w->dts, stream))) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if ((rc = hip_check(heat_launch_dt_fill(w->dts, nsteps, s), "heat_dt"))) return rc;
+  if (integrator != MFVM_INT_EULER) {
+    for (int n = 0; n < nsteps; ++n)
+      if ((rc = step_impl(g, ph, w, integrator, w->dts + n, 0.0, nullptr, n == nsteps - 1, stream))) return rc;
+    return MFVM_OK;
   }
-  if (nsteps > 0 && (rc = mfvm_heat_dt(g, p, cfl, w->dts + nsteps, stream))) return rc;
+  // Euler: the steps alternate between U and S1 (time_integrators.py:43-47
+  // per step: ghosts of the input, then out = in + dt k) instead of copying
+  // S1 back into U after each step.
+  void* buf[2] = {w->U, w->S1};
+  int cur = 0;
+  for (int n = 0; n < nsteps; ++n) {
+    void* in = buf[cur];
+    void* out = buf[1 - cur];
+    if ((rc = mfvm_apply_bcs(g, 1, ph.bc, ph.dtype, in, in, stream))) return rc;
+    StageIO io = {};
+    io.U0 = in;
+    io.out = out;
+    io.kacc = w->kacc;
+    io.update = MFVM_UPD_EULER;
+    io.acc = MFVM_ACC_NONE;
+    io.dt_dev = w->dts + n;
+    if ((rc = heat_stage_op(&x, in, nullptr, io, nullptr, stream))) return rc;
+    cur = 1 - cur;
+  }
+  // The reference's U after the last step: the ghosts filled from the last
+  // input, the interior updated.
+  if (nsteps > 0 && cur == 1) {
+    if ((rc = mfvm_copy_interior(g, 1, p->dtype, w->S1, w->U, stream))) return rc;
+  } else if (nsteps > 0) {
+    if ((rc = mfvm_apply_bcs(g, 1, ph.bc, ph.dtype, w->S1, w->U, stream))) return rc;
+  }
   return MFVM_OK;
 }
 
