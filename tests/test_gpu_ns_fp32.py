@@ -123,3 +123,38 @@ def test_split_stages_equal_fused_stages_2d(monkeypatch):
     torch.cuda.synchronize()
     assert torch.equal(d1, d2)
     assert torch.equal(U.buf, V.buf)
+
+
+def test_rt256_full_size_split_fused_and_compensated(monkeypatch):
+    """The benchmark's workload at its full size (Rayleigh-Taylor 256^3),
+    where the oracle would take hours: properties that do not depend on the
+    size.  Split stages equal the fused stage bit for bit over 3 SSPRK3 steps
+    (dt sequence included), and 20 FP32-compensated steps stay within the
+    L1 <= 1e-6 bar of the FP64 result."""
+    from mfvm import _lib
+    n = (256, 256, 256)
+    ns, grid, U, params, schemes, bcs = _setup(n, "rt")
+    cfg = {"nx1": n[0], "nx2": n[1], "nx3": n[2]}
+    V = ns.allocate_u(cfg)
+    V.buf.copy_(U.buf)
+    U0 = U.buf.clone()
+    print("rt256: split vs fused, 3 steps", flush=True)
+    monkeypatch.setenv("MFVM_SPLIT", "1")
+    d1 = ns.advance(U, grid, params, schemes, bcs, _lib.INT_SSPRK3, 0.4, 3)
+    monkeypatch.setenv("MFVM_SPLIT", "0")
+    d2 = ns.advance(V, grid, params, schemes, bcs, _lib.INT_SSPRK3, 0.4, 3)
+    torch.cuda.synchronize()
+    assert torch.equal(d1, d2)
+    assert torch.equal(U.buf, V.buf)
+    del V
+    print("rt256: compensated vs fp64, 20 steps", flush=True)
+    monkeypatch.setenv("MFVM_SPLIT", "1")
+    U.buf.copy_(U0)
+    Uc = ns.to_compensated(U)
+    ns.advance(U, grid, params, schemes, bcs, _lib.INT_SSPRK3, 0.4, 20)
+    ns.advance_compensated(Uc, grid, params, schemes, bcs, _lib.INT_SSPRK3, 0.4, 20)
+    torch.cuda.synchronize()
+    err = _errors(U.buf, ns.compensated_value(Uc), ns.VARS)
+    for name in ns.VARS:
+        print(f"rt256 {name}: comp L1 {err[name][0]:.3e} Linf {err[name][1]:.3e}", flush=True)
+        assert err[name][0] <= 1e-6, (name, err[name])
