@@ -430,7 +430,10 @@ int mfvm_heat_dt(const mfvm_grid* g, const mfvm_heat_params* p, double cfl, doub
 /* One integrator call (time_integrators.py with heat.compute_rhs). */
 int mfvm_heat_step(const mfvm_grid* g, const mfvm_heat_params* p, const mfvm_ns_work* w,
                    int32_t integrator, double dt, void* stream);
-/* nsteps steps of solver.py:137-151; dts needs nsteps+1 entries. */
+/* nsteps steps of solver.py:137-151; dts needs nsteps+1 entries.  The heat
+ * dt depends on the parameters and the grid only: it is computed once and
+ * written to every entry.  Euler steps alternate w->U and w->S1; w->U ends as
+ * the reference leaves U. */
 int mfvm_heat_advance(const mfvm_grid* g, const mfvm_heat_params* p, const mfvm_ns_work* w,
                       int32_t integrator, double cfl, int32_t nsteps, void* stream);
 
