@@ -382,10 +382,15 @@ class Trainer:
         self.wf = (1.0 / std ** 2).float().to(self.net.device)
         return std
 
-    def train(self, X, T, epochs=10, batch=16, lr=1e-3, log=print):
+    def train(self, X, T, epochs=10, batch=16, lr=1e-3, cosine=False, log=print):
+        """Adam over shuffled minibatches; ``cosine``: the step size follows
+        lr * (1 + cos(pi * epoch / epochs)) / 2 down to zero."""
         n = X.shape[0]
         hist = []
+        base = lr
         for ep in range(epochs):
+            if cosine:
+                lr = base * 0.5 * (1.0 + math.cos(math.pi * ep / epochs))
             perm = torch.randperm(n, device=X.device)
             tot = torch.zeros(1, dtype=torch.float32, device=X.device)
             nb = 0

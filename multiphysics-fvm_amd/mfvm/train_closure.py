@@ -39,21 +39,36 @@ def main(argv=None):
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--batch", type=int, default=32)
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--cosine", action="store_true", help="cosine decay of the Adam step size")
     ap.add_argument("--t-end", type=float, default=2.0)
     ap.add_argument("--out", default=DEFAULT_WEIGHTS)
     ap.add_argument("--report", default="")
+    ap.add_argument("--eval-only", default="", metavar="WEIGHTS",
+                    help="score saved weights on the held-out seeds without training")
     args = ap.parse_args(argv)
 
     def log(msg):
         print(msg, flush=True)
 
     t0 = time.perf_counter()
+    if args.eval_only:
+        net = closure.FluxCorrectionCNN.load(args.eval_only)
+        rep = closure.evaluate(net, list(range(5000, 5000 + args.heldout)), PARAMS, SCHEMES, BCS,
+                               t_end=args.t_end, log=log)
+        rep["weights"] = os.path.basename(args.eval_only)
+        rep["total_s"] = time.perf_counter() - t0
+        line = json.dumps(rep)
+        print(line, flush=True)
+        if args.report:
+            with open(args.report, "w") as f:
+                f.write(line + "\n")
+        return 0
     net = closure.FluxCorrectionCNN(seed=0)
     tr = closure.Trainer(net, PARAMS, SCHEMES, BCS, n_coarse=128, factor=4)
     X, T = tr.generate(list(range(1000, 1000 + args.train)), t_end=args.t_end, pairs=args.pairs, log=log)
     std = tr.set_scales(T)
     log(f"data: {X.shape[0]} samples, target std {std.tolist()}, {time.perf_counter() - t0:.0f} s")
-    hist = tr.train(X, T, epochs=args.epochs, batch=args.batch, lr=args.lr, log=log)
+    hist = tr.train(X, T, epochs=args.epochs, batch=args.batch, lr=args.lr, cosine=args.cosine, log=log)
     del X, T
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     net.save(args.out)
