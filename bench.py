@@ -79,7 +79,7 @@ def bc_rows():
     return out
 
 
-def cpu_baseline(n=80, steps=2):
+def cpu_baseline(n=120, steps=2):
     """The NumPy oracle on a bounded sample (same scheme, n^3, `steps` steps)."""
     from oracle import grid as og
     from oracle import navier_stokes as ons
