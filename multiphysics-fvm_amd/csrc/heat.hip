@@ -1,14 +1,14 @@
 // Heat equation kernels (reference src/physics/heat.py).
 //
-// One thread per interior cell.  The Fourier flux phi_d = -k dT/dx_d is a
-// centred difference of T = e/(rho cp) on interior cells and exactly zero on
-// ghost cells (the reference zero-pads the closure, heat.py:95); only phi_d
-// is reconstructed along d (the face normal picks that component).  The face
-// flux is the centred mean of the two reconstructed values
-// (diffusive_solvers.py:28), F = 0.0 + Fd (heat.py:446-448), and
-// rhs = -sum(F.S)/V + q (heat.py:471).  Neighbouring phi values are
-// recomputed per thread (a 2-deep stencil of T) rather than staged: the
-// update is bandwidth-bound and the recomputation is cheap.
+// One thread per interior cell of a 64 x 4 tile of a plane.  The Fourier
+// flux phi_d = -k dT/dx_d is a centred difference of T = e/(rho cp) on
+// interior cells and exactly zero on ghost cells (the reference zero-pads the
+// closure, heat.py:95); only phi_d is reconstructed along d (the face normal
+// picks that component).  The face flux is the centred mean of the two
+// reconstructed values (diffusive_solvers.py:28), F = 0.0 + Fd
+// (heat.py:446-448), and rhs = -sum(F.S)/V + q (heat.py:471).  T of the tile
+// and a 3-cell ring is staged in LDS and phi_1 / phi_2 are formed once per
+// position; the x3 stencil is read from memory per thread.
 #include "common.h"
 
 namespace mfvm {

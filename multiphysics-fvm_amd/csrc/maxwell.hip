@@ -4,11 +4,13 @@
 // maxwell_res_kernel: residuals rB = div B and rD = div D - rho on interior
 // cells (maxwell.py:491-508), with the edge padding of maxwell.py:511-512
 // written into the face ghosts.
-// maxwell_stage_kernel: one thread per interior cell; MUSCL/PC states of the
-// six fields on both faces along each direction, curl flux of H = B/mu and
-// E = D/eps (maxwell.py:137-181), Rusanov or HLL with the light-speed wave
-// set, geometric source (:366-371), Ohm current (:415-420), cleaning source
-// (:514-528), divergence and the fused stage update.
+// maxwell_stage_kernel (and its split form maxwell_xy_kernel +
+// maxwell_z_kernel): a 64 x 4 tile of (x1, x2) columns marching in x3;
+// MUSCL/PC states of the six fields on both faces along each direction, curl
+// flux of H = B/mu and E = D/eps (maxwell.py:137-181), Rusanov or HLL with
+// the light-speed wave set, geometric source (:366-371), Ohm current
+// (:415-420), cleaning source (:514-528), divergence and the fused stage
+// update.  Each face flux is computed once.
 #include "common.h"
 
 namespace mfvm {
