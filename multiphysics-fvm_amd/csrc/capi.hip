@@ -671,6 +671,7 @@ static int closure_step(const mfvm_grid* g, const mfvm_ns_params* p, const mfvm_
     st.c0 = seq[q].c0;
     st.c1 = seq[q].c1;
     st.dt_dev = cw->dt_cur;
+    st.xy = w->xy;  // split stages where the caller provides the scratch (same results)
     if (e == hipSuccess) e = launch_stage<double>(g, p, &st, s);
     if (!net) continue;
     if (e == hipSuccess)
