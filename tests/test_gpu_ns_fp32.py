@@ -158,3 +158,28 @@ def test_rt256_full_size_split_fused_and_compensated(monkeypatch):
     for name in ns.VARS:
         print(f"rt256 {name}: comp L1 {err[name][0]:.3e} Linf {err[name][1]:.3e}", flush=True)
         assert err[name][0] <= 1e-6, (name, err[name])
+
+
+@pytest.mark.parametrize("comp", [False, True])
+def test_split_stages_with_runtime_tile_equal_fused(comp, monkeypatch):
+    """The x1/x2 kernel's runtime-tile form (any tile but 64 x 4, and arrays
+    above 4 GiB) gives the fused stage's results bit for bit as well."""
+    from mfvm import _lib
+    ns, grid, U, params, schemes, bcs = _setup((40, 40, 40), "rt")
+    V = ns.allocate_u({"nx1": 40, "nx2": 40, "nx3": 40})
+    V.buf.copy_(U.buf)
+    if comp:
+        U, V = ns.to_compensated(U), ns.to_compensated(V)
+        run = ns.advance_compensated
+    else:
+        run = ns.advance
+    tile = (32, 8, 8)
+    monkeypatch.setenv("MFVM_SPLIT", "1")
+    monkeypatch.setenv("MFVM_SPLIT_COMP", "1")
+    d1 = run(U, grid, params, schemes, bcs, _lib.INT_SSPRK3, 0.4, 4, tile=tile)
+    monkeypatch.setenv("MFVM_SPLIT", "0")
+    monkeypatch.setenv("MFVM_SPLIT_COMP", "0")
+    d2 = run(V, grid, params, schemes, bcs, _lib.INT_SSPRK3, 0.4, 4, tile=tile)
+    torch.cuda.synchronize()
+    assert torch.equal(d1, d2)
+    assert torch.equal(U.buf, V.buf)
