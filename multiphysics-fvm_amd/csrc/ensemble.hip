@@ -158,7 +158,7 @@ __global__ __launch_bounds__(kEnsBlock) void ens_xy_kernel(const EnsMem* __restr
   }
 }
 
-template <int LIM, int RS, bool UNIF, class R, bool COMP>
+template <int LIM, int RS, bool UNIF, class R, bool COMP, int FIN = 2>
 MFVM_DEV void ens_z_loop(const EnsMem* __restrict__ mem, const StageArgs* __restrict__ st,
                          const EnsWork* __restrict__ work, long long nwork, int remap) {
   for (long long v = blockIdx.x; v < nwork; v += gridDim.x) {
@@ -172,7 +172,7 @@ MFVM_DEV void ens_z_loop(const EnsMem* __restrict__ mem, const StageArgs* __rest
     tc.y = (w.t / e.ntx) % e.nty;
     tc.z = w.t / (e.ntx * e.nty);
     tc.slot = a.partials ? a.partials + 2 * (long long)w.t : nullptr;
-    ns_stage_body<LIM, RS, UNIF, R, COMP, true>(a, tc, ptr_src<R>(a));
+    ns_stage_body<LIM, RS, UNIF, R, COMP, true, PtrSrc<R>, FIN>(a, tc, ptr_src<R>(a));
     __syncthreads();
   }
 }
@@ -185,12 +185,14 @@ __global__ __launch_bounds__(kEnsBlock) void ens_z_kernel(const EnsMem* __restri
   ens_z_loop<LIM, RS, UNIF, double, false>(mem, st, work, nwork, remap);
 }
 
-template <int LIM, int RS, bool UNIF>
-__global__ __launch_bounds__(kEnsBlock, 2) void ens_z_comp_kernel(const EnsMem* __restrict__ mem,
-                                                                  const StageArgs* __restrict__ st,
-                                                                  const EnsWork* __restrict__ work,
-                                                                  long long nwork, int remap) {
-  ens_z_loop<LIM, RS, UNIF, float, true>(mem, st, work, nwork, remap);
+// FIN: the final stage (float64 accumulation, CFL partials) or not, as in
+// ns_z_comp_kernel.
+template <int LIM, int RS, bool UNIF, bool FIN>
+__global__ __launch_bounds__(kEnsBlock, FIN ? 2 : 3) void ens_z_comp_kernel(const EnsMem* __restrict__ mem,
+                                                                            const StageArgs* __restrict__ st,
+                                                                            const EnsWork* __restrict__ work,
+                                                                            long long nwork, int remap) {
+  ens_z_loop<LIM, RS, UNIF, float, true, FIN ? 1 : 0>(mem, st, work, nwork, remap);
 }
 
 template <bool UNIF, class R>
@@ -404,14 +406,18 @@ static unsigned grid_for(long long nitems, int persist_blocks) {
 }
 
 template <int LIM, int RS, bool UNIF>
-static void launch_ens_stage_t(const EnsRun& r, const StageArgs* st, hipStream_t s) {
+static void launch_ens_stage_t(const EnsRun& r, const StageArgs* st, bool fin, hipStream_t s) {
   const unsigned g = grid_for(r.nsw, r.persist_blocks);
   if (r.split) {
     if (r.comp) {
       hipLaunchKernelGGL((ens_xy_kernel<LIM, RS, UNIF, float>), dim3(g), dim3(kEnsBlock), r.lds_stage, s, r.d_mem,
                          st, r.d_sw, r.nsw, r.remap);
-      hipLaunchKernelGGL((ens_z_comp_kernel<LIM, RS, UNIF>), dim3(g), dim3(kEnsBlock), r.lds_stage, s, r.d_mem,
-                         st, r.d_sw, r.nsw, r.remap);
+      if (fin)
+        hipLaunchKernelGGL((ens_z_comp_kernel<LIM, RS, UNIF, true>), dim3(g), dim3(kEnsBlock), r.lds_stage, s,
+                           r.d_mem, st, r.d_sw, r.nsw, r.remap);
+      else
+        hipLaunchKernelGGL((ens_z_comp_kernel<LIM, RS, UNIF, false>), dim3(g), dim3(kEnsBlock), r.lds_stage, s,
+                           r.d_mem, st, r.d_sw, r.nsw, r.remap);
     } else {
       hipLaunchKernelGGL((ens_xy_kernel<LIM, RS, UNIF, double>), dim3(g), dim3(kEnsBlock), r.lds_stage, s,
                          r.d_mem, st, r.d_sw, r.nsw, r.remap);
@@ -429,24 +435,25 @@ static void launch_ens_stage_t(const EnsRun& r, const StageArgs* st, hipStream_t
 }
 
 template <int LIM, int RS>
-static void launch_ens_stage_u(const EnsRun& r, const StageArgs* st, hipStream_t s) {
-  if (r.unif) launch_ens_stage_t<LIM, RS, true>(r, st, s);
-  else launch_ens_stage_t<LIM, RS, false>(r, st, s);
+static void launch_ens_stage_u(const EnsRun& r, const StageArgs* st, bool fin, hipStream_t s) {
+  if (r.unif) launch_ens_stage_t<LIM, RS, true>(r, st, fin, s);
+  else launch_ens_stage_t<LIM, RS, false>(r, st, fin, s);
 }
 
 template <int LIM>
-static void launch_ens_stage_l(const EnsRun& r, const StageArgs* st, hipStream_t s) {
-  if (r.rs == MFVM_RS_HLL) launch_ens_stage_u<LIM, MFVM_RS_HLL>(r, st, s);
-  else launch_ens_stage_u<LIM, MFVM_RS_RUSANOV>(r, st, s);
+static void launch_ens_stage_l(const EnsRun& r, const StageArgs* st, bool fin, hipStream_t s) {
+  if (r.rs == MFVM_RS_HLL) launch_ens_stage_u<LIM, MFVM_RS_HLL>(r, st, fin, s);
+  else launch_ens_stage_u<LIM, MFVM_RS_RUSANOV>(r, st, fin, s);
 }
 
 static void launch_ens_stage(const EnsRun& r, int q, hipStream_t s) {
   const StageArgs* st = r.d_st + (size_t)q * r.n;
+  const bool fin = q == r.nst - 1;  // the compensated sequences finish in the last stage
   switch (r.lim) {
-    case MFVM_LIM_NONE: launch_ens_stage_l<MFVM_LIM_NONE>(r, st, s); break;
-    case MFVM_LIM_MINMOD: launch_ens_stage_l<MFVM_LIM_MINMOD>(r, st, s); break;
-    case MFVM_LIM_MC: launch_ens_stage_l<MFVM_LIM_MC>(r, st, s); break;
-    default: launch_ens_stage_l<MFVM_LIM_VANLEER>(r, st, s); break;
+    case MFVM_LIM_NONE: launch_ens_stage_l<MFVM_LIM_NONE>(r, st, fin, s); break;
+    case MFVM_LIM_MINMOD: launch_ens_stage_l<MFVM_LIM_MINMOD>(r, st, fin, s); break;
+    case MFVM_LIM_MC: launch_ens_stage_l<MFVM_LIM_MC>(r, st, fin, s); break;
+    default: launch_ens_stage_l<MFVM_LIM_VANLEER>(r, st, fin, s); break;
   }
 }
 

@@ -83,11 +83,11 @@ __global__ __launch_bounds__(256) void ns_z_kernel(StageArgs a) {
   ns_stage_body<LIM, RS, UNIF, double, false, true>(a, grid_tile(a), ptr_src<double>(a));
 }
 
-// Two waves per SIMD without spills: three waves (168 VGPRs) spill the
-// batched finalize loads to scratch and run slower.
-template <int LIM, int RS, bool UNIF>
-__global__ __launch_bounds__(256, 2) void ns_z_comp_kernel(StageArgs a) {
-  ns_stage_body<LIM, RS, UNIF, float, true, true>(a, grid_tile(a), ptr_src<float>(a));
+// FIN: the final stage (float64 accumulation and CFL partials) at two waves
+// per SIMD; the other stages leave that code out and fit three (144 VGPRs).
+template <int LIM, int RS, bool UNIF, bool FIN>
+__global__ __launch_bounds__(256, FIN ? 2 : 3) void ns_z_comp_kernel(StageArgs a) {
+  ns_stage_body<LIM, RS, UNIF, float, true, true, PtrSrc<float>, FIN ? 1 : 0>(a, grid_tile(a), ptr_src<float>(a));
 }
 
 // Standalone CFL partials (navier_stokes.py:512-583), same grid as the stage.
@@ -179,8 +179,11 @@ static hipError_t launch_stage_t(const StageArgs& a, bool unif, dim3 grd, dim3 b
     t_end(tx, s);
     const int tz = t_begin(MFVM_TK_OTHER, s);
     if (comp) {
-      if (unif) hipLaunchKernelGGL((ns_z_comp_kernel<LIM, RS, true>), grd, blk, red, s, a);
-      else hipLaunchKernelGGL((ns_z_comp_kernel<LIM, RS, false>), grd, blk, red, s, a);
+      const bool fin = a.comp >= MFVM_COMP_FINAL_SET;
+      if (unif && fin) hipLaunchKernelGGL((ns_z_comp_kernel<LIM, RS, true, true>), grd, blk, red, s, a);
+      else if (unif) hipLaunchKernelGGL((ns_z_comp_kernel<LIM, RS, true, false>), grd, blk, red, s, a);
+      else if (fin) hipLaunchKernelGGL((ns_z_comp_kernel<LIM, RS, false, true>), grd, blk, red, s, a);
+      else hipLaunchKernelGGL((ns_z_comp_kernel<LIM, RS, false, false>), grd, blk, red, s, a);
     } else {
       if (unif) hipLaunchKernelGGL((ns_z_kernel<LIM, RS, true>), grd, blk, red, s, a);
       else hipLaunchKernelGGL((ns_z_kernel<LIM, RS, false>), grd, blk, red, s, a);

@@ -765,7 +765,10 @@ MFVM_DEV void inplane_div(const StageArgs& a, const S& src, R* sm, int BX, int B
 // ZONLY: the in-plane part of the divergence comes from a.xy (written by the
 // x1/x2 kernel of a split stage) instead of being computed here.  src reads
 // a.U and a.C (PtrSrc or BufSrc).
-template <int LIM, int RS, bool UNIF, class R, bool COMP, bool ZONLY, class S>
+// FIN (compensated stages): 2 -- whether this is the final stage is read
+// from a.comp; 0 / 1 -- known at compile time (a.comp must agree), which
+// leaves the float64 accumulation and CFL code out of the other stages.
+template <int LIM, int RS, bool UNIF, class R, bool COMP, bool ZONLY, class S, int FIN = 2>
 MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl, const S& src) {
   extern __shared__ double smem_raw[];
   R* sm = reinterpret_cast<R*>(smem_raw);
@@ -932,7 +935,7 @@ MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl, const S& src)
         R* lo = static_cast<R*>(a.lo);
         R* dv = static_cast<R*>(a.dinc);
         R* sn = static_cast<R*>(a.snext);
-        const bool fin = a.comp >= MFVM_COMP_FINAL_SET;
+        const bool fin = FIN == 2 ? a.comp >= MFVM_COMP_FINAL_SET : FIN == 1;
         const bool needd = a.comp == MFVM_COMP_SSP || a.comp == MFVM_COMP_FINAL_SSP;
         const bool needk = a.comp == MFVM_COMP_FINAL_ACC6 || a.acc == MFVM_ACC_ADD;
         double nw[5];
