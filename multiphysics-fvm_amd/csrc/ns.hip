@@ -65,8 +65,16 @@ __global__ __launch_bounds__(256, 2) void ns_stage_comp_kernel(StageArgs a) {
 // and far fewer scalar registers spill into vector lanes (float: 87 instead
 // of 110 VGPRs, five waves per SIMD instead of four).  Other tile shapes and
 // larger arrays take FIXED = false.
-template <int LIM, int RS, bool UNIF, class R, bool FIXED>
+//
+// P2 (with FIXED): every cell length and the volume are one power of two
+// (host-checked), so the exact reciprocal multiplies are known to apply at
+// compile time; without P2 each use tests the flag and branches.
+template <int LIM, int RS, bool UNIF, class R, bool FIXED, bool P2 = false>
 __global__ __launch_bounds__(256, FIXED && sizeof(R) == 8 ? 3 : 1) void ns_xy_kernel(StageArgs a) {
+  if constexpr (P2) {
+    a.geo.lpow2[0] = a.geo.lpow2[1] = a.geo.lpow2[2] = 1;
+    a.geo.vpow2 = 1;
+  }
   TileCtx t = grid_tile(a);
   if constexpr (FIXED) {
     t.bx = 64;
@@ -78,15 +86,23 @@ __global__ __launch_bounds__(256, FIXED && sizeof(R) == 8 ? 3 : 1) void ns_xy_ke
 }
 
 // (Buffer-resource reads measure the same here as pointer reads.)
-template <int LIM, int RS, bool UNIF>
+template <int LIM, int RS, bool UNIF, bool P2>
 __global__ __launch_bounds__(256) void ns_z_kernel(StageArgs a) {
+  if constexpr (P2) {
+    a.geo.lpow2[0] = a.geo.lpow2[1] = a.geo.lpow2[2] = 1;
+    a.geo.vpow2 = 1;
+  }
   ns_stage_body<LIM, RS, UNIF, double, false, true>(a, grid_tile(a), ptr_src<double>(a));
 }
 
 // FIN: the final stage (float64 accumulation and CFL partials) at two waves
 // per SIMD; the other stages leave that code out and fit three (144 VGPRs).
-template <int LIM, int RS, bool UNIF, bool FIN>
+template <int LIM, int RS, bool UNIF, bool FIN, bool P2>
 __global__ __launch_bounds__(256, FIN ? 2 : 3) void ns_z_comp_kernel(StageArgs a) {
+  if constexpr (P2) {
+    a.geo.lpow2[0] = a.geo.lpow2[1] = a.geo.lpow2[2] = 1;
+    a.geo.vpow2 = 1;
+  }
   ns_stage_body<LIM, RS, UNIF, float, true, true, PtrSrc<float>, FIN ? 1 : 0>(a, grid_tile(a), ptr_src<float>(a));
 }
 
@@ -171,8 +187,10 @@ static hipError_t launch_stage_t(const StageArgs& a, bool unif, dim3 grd, dim3 b
     ax.partials = nullptr;
     const dim3 gx(grd.x, grd.y, (a.lay.n3 + ax.kz - 1) / ax.kz);
     const bool fixed = blk.x == 64 && blk.y == 4 && buf_src_ok(a.lay, sizeof(R));
+    const bool p2 = a.geo.lpow2[0] && a.geo.lpow2[1] && a.geo.lpow2[2] && a.geo.vpow2;
     const int tx = t_begin(MFVM_TK_STAGE, s);
-    if (unif && fixed) hipLaunchKernelGGL((ns_xy_kernel<LIM, RS, true, R, true>), gx, blk, lds, s, ax);
+    if (unif && fixed && p2) hipLaunchKernelGGL((ns_xy_kernel<LIM, RS, true, R, true, true>), gx, blk, lds, s, ax);
+    else if (unif && fixed) hipLaunchKernelGGL((ns_xy_kernel<LIM, RS, true, R, true>), gx, blk, lds, s, ax);
     else if (unif) hipLaunchKernelGGL((ns_xy_kernel<LIM, RS, true, R, false>), gx, blk, lds, s, ax);
     else if (fixed) hipLaunchKernelGGL((ns_xy_kernel<LIM, RS, false, R, true>), gx, blk, lds, s, ax);
     else hipLaunchKernelGGL((ns_xy_kernel<LIM, RS, false, R, false>), gx, blk, lds, s, ax);
@@ -180,13 +198,16 @@ static hipError_t launch_stage_t(const StageArgs& a, bool unif, dim3 grd, dim3 b
     const int tz = t_begin(MFVM_TK_OTHER, s);
     if (comp) {
       const bool fin = a.comp >= MFVM_COMP_FINAL_SET;
-      if (unif && fin) hipLaunchKernelGGL((ns_z_comp_kernel<LIM, RS, true, true>), grd, blk, red, s, a);
-      else if (unif) hipLaunchKernelGGL((ns_z_comp_kernel<LIM, RS, true, false>), grd, blk, red, s, a);
-      else if (fin) hipLaunchKernelGGL((ns_z_comp_kernel<LIM, RS, false, true>), grd, blk, red, s, a);
-      else hipLaunchKernelGGL((ns_z_comp_kernel<LIM, RS, false, false>), grd, blk, red, s, a);
+      if (unif && p2 && fin) hipLaunchKernelGGL((ns_z_comp_kernel<LIM, RS, true, true, true>), grd, blk, red, s, a);
+      else if (unif && p2) hipLaunchKernelGGL((ns_z_comp_kernel<LIM, RS, true, false, true>), grd, blk, red, s, a);
+      else if (unif && fin) hipLaunchKernelGGL((ns_z_comp_kernel<LIM, RS, true, true, false>), grd, blk, red, s, a);
+      else if (unif) hipLaunchKernelGGL((ns_z_comp_kernel<LIM, RS, true, false, false>), grd, blk, red, s, a);
+      else if (fin) hipLaunchKernelGGL((ns_z_comp_kernel<LIM, RS, false, true, false>), grd, blk, red, s, a);
+      else hipLaunchKernelGGL((ns_z_comp_kernel<LIM, RS, false, false, false>), grd, blk, red, s, a);
     } else {
-      if (unif) hipLaunchKernelGGL((ns_z_kernel<LIM, RS, true>), grd, blk, red, s, a);
-      else hipLaunchKernelGGL((ns_z_kernel<LIM, RS, false>), grd, blk, red, s, a);
+      if (unif && p2) hipLaunchKernelGGL((ns_z_kernel<LIM, RS, true, true>), grd, blk, red, s, a);
+      else if (unif) hipLaunchKernelGGL((ns_z_kernel<LIM, RS, true, false>), grd, blk, red, s, a);
+      else hipLaunchKernelGGL((ns_z_kernel<LIM, RS, false, false>), grd, blk, red, s, a);
     }
     t_end(tz, s);
     return hipGetLastError();
