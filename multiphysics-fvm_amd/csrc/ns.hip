@@ -140,6 +140,8 @@ long long ns_nblocks(const mfvm_grid* g, const mfvm_ns_params* p) {
   return (long long)gr.x * gr.y * gr.z;
 }
 
+constexpr int CLOSURE_KZ_F64 = 32, CLOSURE_KZ_F32 = 16;
+
 template <class R>
 hipError_t launch_closure(const mfvm_grid* g, const mfvm_ns_params* p, const void* U, void* C,
                           hipStream_t s) {
@@ -154,7 +156,10 @@ hipError_t launch_closure(const mfvm_grid* g, const mfvm_ns_params* p, const voi
     const dim3 grd((g->n[0] + 2 + 63) / 64, (g->n[1] + 2 + 3) / 4, g->n[2] + 2);
     hipLaunchKernelGGL((ns_closure_p_kernel<R>), grd, blk, 0, s, lay, q, u, c);
   } else {
-    const int kz = g->n[2] < 32 ? g->n[2] : 32;
+    // x3 planes per workgroup: the float32 kernel (no prefetch) measures
+    // 13% faster with chunks of 16 than of 32 at 256^3
+    const int kzz = sizeof(R) == sizeof(float) ? CLOSURE_KZ_F32 : CLOSURE_KZ_F64;
+    const int kz = g->n[2] < kzz ? g->n[2] : kzz;
     const dim3 grd((g->n[0] + 63) / 64, (g->n[1] + 3) / 4, (g->n[2] + kz - 1) / kz);
     const size_t lds = 3 * (4 + 2) * (64 + 2) * sizeof(R);
     const int remap = xcd_remap_enabled();
