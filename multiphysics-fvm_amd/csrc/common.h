@@ -191,6 +191,9 @@ inline ILayout make_ilayout(int n1, int n2, int n3) {
 
 Geo make_geo(const mfvm_grid* g);
 
+// Every cell length and the volume are one power of two.
+inline bool pow2_geo(const Geo& g) { return g.lpow2[0] && g.lpow2[1] && g.lpow2[2] && g.vpow2; }
+
 // Runge-Kutta stage update of one value (time_integrators.py), k = rhs.
 //   EULER: U + dt*k                 BASE: U0 + (c0*dt)*k
 //   SSP:   c0*U0 + c1*(U + dt*k)     ACC6: U0 + (dt/6)*(kacc + k)

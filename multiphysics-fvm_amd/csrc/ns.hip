@@ -41,21 +41,32 @@ __global__ void ns_closure_p_kernel(Layout lay, NSPar p, const R* __restrict__ U
   C[c] = closure_p(p, rho, v[0], v[1], v[2], U[c + 4 * fs]);
 }
 
-template <bool UNIF, class R>
+// P2 (every kernel below): every cell length and the volume are one power of
+// two (host-checked, pow2_geo), so the exact reciprocal multiplies are known
+// to apply at compile time instead of being tested at each use.
+MFVM_DEV void set_pow2(Geo& g) {
+  g.lpow2[0] = g.lpow2[1] = g.lpow2[2] = 1;
+  g.vpow2 = 1;
+}
+
+template <bool UNIF, class R, bool P2>
 __global__ __launch_bounds__(256) void ns_closure_visc_kernel(Layout lay, Geo geo, NSPar p,
                                                               const R* __restrict__ U,
                                                               R* __restrict__ C, int kz, int remap) {
+  if constexpr (P2) set_pow2(geo);
   closure_march_body<true, UNIF, R>(lay, geo, p, U, C, kz, tile_id(remap != 0));
 }
 
-template <int LIM, int RS, bool UNIF, class R>
+template <int LIM, int RS, bool UNIF, class R, bool P2>
 __global__ __launch_bounds__(256) void ns_stage_kernel(StageArgs a) {
+  if constexpr (P2) set_pow2(a.geo);
   ns_stage_body<LIM, RS, UNIF, R, false, false>(a, grid_tile(a), ptr_src<R>(a));
 }
 
 // Compensated FP32 stage: half the registers of the FP64 kernel, two waves per SIMD.
-template <int LIM, int RS, bool UNIF>
+template <int LIM, int RS, bool UNIF, bool P2>
 __global__ __launch_bounds__(256, 2) void ns_stage_comp_kernel(StageArgs a) {
+  if constexpr (P2) set_pow2(a.geo);
   ns_stage_body<LIM, RS, UNIF, float, true, false>(a, grid_tile(a), ptr_src<float>(a));
 }
 
@@ -66,15 +77,10 @@ __global__ __launch_bounds__(256, 2) void ns_stage_comp_kernel(StageArgs a) {
 // of 110 VGPRs, five waves per SIMD instead of four).  Other tile shapes and
 // larger arrays take FIXED = false.
 //
-// P2 (with FIXED): every cell length and the volume are one power of two
-// (host-checked), so the exact reciprocal multiplies are known to apply at
-// compile time; without P2 each use tests the flag and branches.
+// P2 (with FIXED): uniform power-of-two geometry, as for the kernels above.
 template <int LIM, int RS, bool UNIF, class R, bool FIXED, bool P2 = false>
 __global__ __launch_bounds__(256, FIXED && sizeof(R) == 8 ? 3 : 1) void ns_xy_kernel(StageArgs a) {
-  if constexpr (P2) {
-    a.geo.lpow2[0] = a.geo.lpow2[1] = a.geo.lpow2[2] = 1;
-    a.geo.vpow2 = 1;
-  }
+  if constexpr (P2) set_pow2(a.geo);
   TileCtx t = grid_tile(a);
   if constexpr (FIXED) {
     t.bx = 64;
@@ -88,10 +94,7 @@ __global__ __launch_bounds__(256, FIXED && sizeof(R) == 8 ? 3 : 1) void ns_xy_ke
 // (Buffer-resource reads measure the same here as pointer reads.)
 template <int LIM, int RS, bool UNIF, bool P2>
 __global__ __launch_bounds__(256) void ns_z_kernel(StageArgs a) {
-  if constexpr (P2) {
-    a.geo.lpow2[0] = a.geo.lpow2[1] = a.geo.lpow2[2] = 1;
-    a.geo.vpow2 = 1;
-  }
+  if constexpr (P2) set_pow2(a.geo);
   ns_stage_body<LIM, RS, UNIF, double, false, true>(a, grid_tile(a), ptr_src<double>(a));
 }
 
@@ -99,10 +102,7 @@ __global__ __launch_bounds__(256) void ns_z_kernel(StageArgs a) {
 // per SIMD; the other stages leave that code out and fit three (144 VGPRs).
 template <int LIM, int RS, bool UNIF, bool FIN, bool P2>
 __global__ __launch_bounds__(256, FIN ? 2 : 3) void ns_z_comp_kernel(StageArgs a) {
-  if constexpr (P2) {
-    a.geo.lpow2[0] = a.geo.lpow2[1] = a.geo.lpow2[2] = 1;
-    a.geo.vpow2 = 1;
-  }
+  if constexpr (P2) set_pow2(a.geo);
   ns_stage_body<LIM, RS, UNIF, float, true, true, PtrSrc<float>, FIN ? 1 : 0>(a, grid_tile(a), ptr_src<float>(a));
 }
 
@@ -163,10 +163,13 @@ hipError_t launch_closure(const mfvm_grid* g, const mfvm_ns_params* p, const voi
     const dim3 grd((g->n[0] + 63) / 64, (g->n[1] + 3) / 4, (g->n[2] + kz - 1) / kz);
     const size_t lds = 3 * (4 + 2) * (64 + 2) * sizeof(R);
     const int remap = xcd_remap_enabled();
-    if (g->uniform)
-      hipLaunchKernelGGL((ns_closure_visc_kernel<true, R>), grd, blk, lds, s, lay, geo, q, u, c, kz, remap);
+    if (g->uniform && pow2_geo(geo))
+      hipLaunchKernelGGL((ns_closure_visc_kernel<true, R, true>), grd, blk, lds, s, lay, geo, q, u, c, kz, remap);
+    else if (g->uniform)
+      hipLaunchKernelGGL((ns_closure_visc_kernel<true, R, false>), grd, blk, lds, s, lay, geo, q, u, c, kz, remap);
     else
-      hipLaunchKernelGGL((ns_closure_visc_kernel<false, R>), grd, blk, lds, s, lay, geo, q, u, c, kz, remap);
+      hipLaunchKernelGGL((ns_closure_visc_kernel<false, R, false>), grd, blk, lds, s, lay, geo, q, u, c, kz,
+                         remap);
   }
   t_end(tk, s);
   return hipGetLastError();
@@ -192,7 +195,7 @@ static hipError_t launch_stage_t(const StageArgs& a, bool unif, dim3 grd, dim3 b
     ax.partials = nullptr;
     const dim3 gx(grd.x, grd.y, (a.lay.n3 + ax.kz - 1) / ax.kz);
     const bool fixed = blk.x == 64 && blk.y == 4 && buf_src_ok(a.lay, sizeof(R));
-    const bool p2 = a.geo.lpow2[0] && a.geo.lpow2[1] && a.geo.lpow2[2] && a.geo.vpow2;
+    const bool p2 = pow2_geo(a.geo);
     const int tx = t_begin(MFVM_TK_STAGE, s);
     if (unif && fixed && p2) hipLaunchKernelGGL((ns_xy_kernel<LIM, RS, true, R, true, true>), gx, blk, lds, s, ax);
     else if (unif && fixed) hipLaunchKernelGGL((ns_xy_kernel<LIM, RS, true, R, true>), gx, blk, lds, s, ax);
@@ -218,15 +221,20 @@ static hipError_t launch_stage_t(const StageArgs& a, bool unif, dim3 grd, dim3 b
     return hipGetLastError();
   }
   const int tk = t_begin(MFVM_TK_STAGE, s);
+  const bool p2 = pow2_geo(a.geo);
   if (a.comp != MFVM_COMP_NONE) {
-    if (unif)
-      hipLaunchKernelGGL((ns_stage_comp_kernel<LIM, RS, true>), grd, blk, lds, s, a);
+    if (unif && p2)
+      hipLaunchKernelGGL((ns_stage_comp_kernel<LIM, RS, true, true>), grd, blk, lds, s, a);
+    else if (unif)
+      hipLaunchKernelGGL((ns_stage_comp_kernel<LIM, RS, true, false>), grd, blk, lds, s, a);
     else
-      hipLaunchKernelGGL((ns_stage_comp_kernel<LIM, RS, false>), grd, blk, lds, s, a);
-  } else if (unif)
-    hipLaunchKernelGGL((ns_stage_kernel<LIM, RS, true, R>), grd, blk, lds, s, a);
+      hipLaunchKernelGGL((ns_stage_comp_kernel<LIM, RS, false, false>), grd, blk, lds, s, a);
+  } else if (unif && p2)
+    hipLaunchKernelGGL((ns_stage_kernel<LIM, RS, true, R, true>), grd, blk, lds, s, a);
+  else if (unif)
+    hipLaunchKernelGGL((ns_stage_kernel<LIM, RS, true, R, false>), grd, blk, lds, s, a);
   else
-    hipLaunchKernelGGL((ns_stage_kernel<LIM, RS, false, R>), grd, blk, lds, s, a);
+    hipLaunchKernelGGL((ns_stage_kernel<LIM, RS, false, R, false>), grd, blk, lds, s, a);
   t_end(tk, s);
   return hipGetLastError();
 }
