@@ -194,6 +194,14 @@ Geo make_geo(const mfvm_grid* g);
 // Every cell length and the volume are one power of two.
 inline bool pow2_geo(const Geo& g) { return g.lpow2[0] && g.lpow2[1] && g.lpow2[2] && g.vpow2; }
 
+// In a kernel specialised for pow2_geo grids (checked on the host): the
+// flags become compile-time constants once the copy is inlined, so the exact
+// reciprocal multiplies apply without a test and a branch at each use.
+MFVM_DEV void set_pow2(Geo& g) {
+  g.lpow2[0] = g.lpow2[1] = g.lpow2[2] = 1;
+  g.vpow2 = 1;
+}
+
 // Runge-Kutta stage update of one value (time_integrators.py), k = rhs.
 //   EULER: U + dt*k                 BASE: U0 + (c0*dt)*k
 //   SSP:   c0*U0 + c1*(U + dt*k)     ACC6: U0 + (dt/6)*(kacc + k)

@@ -83,7 +83,9 @@ __global__ __launch_bounds__(kEnsBlock) void ens_bc_kernel(const EnsMem* __restr
   dst[var * e.lay.fs + e.lay.at(i, j, k)] = shell_value<R>(e.bca, var, src, i, j, k);
 }
 
-template <bool VISC, bool UNIF, class R>
+// P2 (these kernels): every member's geometry is uniform powers of two
+// (set_pow2).
+template <bool VISC, bool UNIF, class R, bool P2 = false>
 __global__ __launch_bounds__(kEnsBlock) void ens_closure_kernel(const EnsMem* __restrict__ mem, NSPar p,
                                                                 const EnsWork* __restrict__ work,
                                                                 long long nwork, int sel, int remap) {
@@ -94,19 +96,22 @@ __global__ __launch_bounds__(kEnsBlock) void ens_closure_kernel(const EnsMem* __
     tl.x = w.t % e.cnx;
     tl.y = (w.t / e.cnx) % e.cny;
     tl.z = w.t / (e.cnx * e.cny);
-    closure_march_body<VISC, UNIF, R>(e.lay, e.geo, p, static_cast<const R*>(e.buf[sel]),
-                                      static_cast<R*>(e.C), e.ckz, tl);
+    Geo geo = e.geo;
+    if constexpr (P2) set_pow2(geo);
+    closure_march_body<VISC, UNIF, R>(e.lay, geo, p, static_cast<const R*>(e.buf[sel]), static_cast<R*>(e.C),
+                                      e.ckz, tl);
     __syncthreads();  // LDS of this tile is read before the next tile writes it
   }
 }
 
-template <int LIM, int RS, bool UNIF, class R, bool COMP>
+template <int LIM, int RS, bool UNIF, class R, bool COMP, bool P2 = false>
 MFVM_DEV void ens_stage_loop(const EnsMem* __restrict__ mem, const StageArgs* __restrict__ st,
                              const EnsWork* __restrict__ work, long long nwork, int remap) {
   for (long long v = blockIdx.x; v < nwork; v += gridDim.x) {
     const EnsWork w = work[ens_item(v, nwork, remap != 0)];
     const EnsMem& e = mem[w.m];
-    const StageArgs& a = st[w.m];
+    StageArgs a = st[w.m];
+    if constexpr (P2) set_pow2(a.geo);
     TileCtx tc;
     tc.bx = e.bx;
     tc.by = e.by;
@@ -119,25 +124,25 @@ MFVM_DEV void ens_stage_loop(const EnsMem* __restrict__ mem, const StageArgs* __
   }
 }
 
-template <int LIM, int RS, bool UNIF>
+template <int LIM, int RS, bool UNIF, bool P2>
 __global__ __launch_bounds__(kEnsBlock) void ens_stage_kernel(const EnsMem* __restrict__ mem,
                                                               const StageArgs* __restrict__ st,
                                                               const EnsWork* __restrict__ work,
                                                               long long nwork, int remap) {
-  ens_stage_loop<LIM, RS, UNIF, double, false>(mem, st, work, nwork, remap);
+  ens_stage_loop<LIM, RS, UNIF, double, false, P2>(mem, st, work, nwork, remap);
 }
 
-template <int LIM, int RS, bool UNIF>
+template <int LIM, int RS, bool UNIF, bool P2>
 __global__ __launch_bounds__(kEnsBlock, 2) void ens_stage_comp_kernel(const EnsMem* __restrict__ mem,
                                                                       const StageArgs* __restrict__ st,
                                                                       const EnsWork* __restrict__ work,
                                                                       long long nwork, int remap) {
-  ens_stage_loop<LIM, RS, UNIF, float, true>(mem, st, work, nwork, remap);
+  ens_stage_loop<LIM, RS, UNIF, float, true, P2>(mem, st, work, nwork, remap);
 }
 
 // Split stage over the work list: the x1/x2 pass of every member's tiles,
 // then the x3 pass (as ns_xy_kernel / ns_z_kernel of the single-grid path).
-template <int LIM, int RS, bool UNIF, class R>
+template <int LIM, int RS, bool UNIF, class R, bool P2>
 __global__ __launch_bounds__(kEnsBlock) void ens_xy_kernel(const EnsMem* __restrict__ mem,
                                                            const StageArgs* __restrict__ st,
                                                            const EnsWork* __restrict__ work, long long nwork,
@@ -145,7 +150,8 @@ __global__ __launch_bounds__(kEnsBlock) void ens_xy_kernel(const EnsMem* __restr
   for (long long v = blockIdx.x; v < nwork; v += gridDim.x) {
     const EnsWork w = work[ens_item(v, nwork, remap != 0)];
     const EnsMem& e = mem[w.m];
-    const StageArgs& a = st[w.m];
+    StageArgs a = st[w.m];
+    if constexpr (P2) set_pow2(a.geo);
     TileCtx tc;
     tc.bx = e.bx;
     tc.by = e.by;
@@ -158,13 +164,14 @@ __global__ __launch_bounds__(kEnsBlock) void ens_xy_kernel(const EnsMem* __restr
   }
 }
 
-template <int LIM, int RS, bool UNIF, class R, bool COMP, int FIN = 2>
+template <int LIM, int RS, bool UNIF, class R, bool COMP, int FIN = 2, bool P2 = false>
 MFVM_DEV void ens_z_loop(const EnsMem* __restrict__ mem, const StageArgs* __restrict__ st,
                          const EnsWork* __restrict__ work, long long nwork, int remap) {
   for (long long v = blockIdx.x; v < nwork; v += gridDim.x) {
     const EnsWork w = work[ens_item(v, nwork, remap != 0)];
     const EnsMem& e = mem[w.m];
-    const StageArgs& a = st[w.m];
+    StageArgs a = st[w.m];
+    if constexpr (P2) set_pow2(a.geo);
     TileCtx tc;
     tc.bx = e.bx;
     tc.by = e.by;
@@ -177,22 +184,22 @@ MFVM_DEV void ens_z_loop(const EnsMem* __restrict__ mem, const StageArgs* __rest
   }
 }
 
-template <int LIM, int RS, bool UNIF>
+template <int LIM, int RS, bool UNIF, bool P2>
 __global__ __launch_bounds__(kEnsBlock) void ens_z_kernel(const EnsMem* __restrict__ mem,
                                                           const StageArgs* __restrict__ st,
                                                           const EnsWork* __restrict__ work, long long nwork,
                                                           int remap) {
-  ens_z_loop<LIM, RS, UNIF, double, false>(mem, st, work, nwork, remap);
+  ens_z_loop<LIM, RS, UNIF, double, false, 2, P2>(mem, st, work, nwork, remap);
 }
 
 // FIN: the final stage (float64 accumulation, CFL partials) or not, as in
 // ns_z_comp_kernel.
-template <int LIM, int RS, bool UNIF, bool FIN>
+template <int LIM, int RS, bool UNIF, bool FIN, bool P2>
 __global__ __launch_bounds__(kEnsBlock, FIN ? 2 : 3) void ens_z_comp_kernel(const EnsMem* __restrict__ mem,
                                                                             const StageArgs* __restrict__ st,
                                                                             const EnsWork* __restrict__ work,
                                                                             long long nwork, int remap) {
-  ens_z_loop<LIM, RS, UNIF, float, true, FIN ? 1 : 0>(mem, st, work, nwork, remap);
+  ens_z_loop<LIM, RS, UNIF, float, true, FIN ? 1 : 0, P2>(mem, st, work, nwork, remap);
 }
 
 template <bool UNIF, class R>
@@ -397,6 +404,7 @@ struct EnsRun {
   int persist_blocks;  // 0: one workgroup per item
   int remap;
   bool split;          // every member has an xy scratch: x1/x2 pass + x3 pass
+  bool p2;             // every member's geometry is uniform powers of two (pow2_geo)
   int in_sel[4];
 };
 
@@ -405,39 +413,41 @@ static unsigned grid_for(long long nitems, int persist_blocks) {
   return (unsigned)nitems;
 }
 
-template <int LIM, int RS, bool UNIF>
+template <int LIM, int RS, bool UNIF, bool P2>
 static void launch_ens_stage_t(const EnsRun& r, const StageArgs* st, bool fin, hipStream_t s) {
   const unsigned g = grid_for(r.nsw, r.persist_blocks);
+  const dim3 grd(g), blk(kEnsBlock);
   if (r.split) {
     if (r.comp) {
-      hipLaunchKernelGGL((ens_xy_kernel<LIM, RS, UNIF, float>), dim3(g), dim3(kEnsBlock), r.lds_stage, s, r.d_mem,
-                         st, r.d_sw, r.nsw, r.remap);
+      hipLaunchKernelGGL((ens_xy_kernel<LIM, RS, UNIF, float, P2>), grd, blk, r.lds_stage, s, r.d_mem, st, r.d_sw,
+                         r.nsw, r.remap);
       if (fin)
-        hipLaunchKernelGGL((ens_z_comp_kernel<LIM, RS, UNIF, true>), dim3(g), dim3(kEnsBlock), r.lds_stage, s,
-                           r.d_mem, st, r.d_sw, r.nsw, r.remap);
+        hipLaunchKernelGGL((ens_z_comp_kernel<LIM, RS, UNIF, true, P2>), grd, blk, r.lds_stage, s, r.d_mem, st,
+                           r.d_sw, r.nsw, r.remap);
       else
-        hipLaunchKernelGGL((ens_z_comp_kernel<LIM, RS, UNIF, false>), dim3(g), dim3(kEnsBlock), r.lds_stage, s,
-                           r.d_mem, st, r.d_sw, r.nsw, r.remap);
+        hipLaunchKernelGGL((ens_z_comp_kernel<LIM, RS, UNIF, false, P2>), grd, blk, r.lds_stage, s, r.d_mem, st,
+                           r.d_sw, r.nsw, r.remap);
     } else {
-      hipLaunchKernelGGL((ens_xy_kernel<LIM, RS, UNIF, double>), dim3(g), dim3(kEnsBlock), r.lds_stage, s,
-                         r.d_mem, st, r.d_sw, r.nsw, r.remap);
-      hipLaunchKernelGGL((ens_z_kernel<LIM, RS, UNIF>), dim3(g), dim3(kEnsBlock), r.lds_stage, s, r.d_mem, st,
-                         r.d_sw, r.nsw, r.remap);
+      hipLaunchKernelGGL((ens_xy_kernel<LIM, RS, UNIF, double, P2>), grd, blk, r.lds_stage, s, r.d_mem, st, r.d_sw,
+                         r.nsw, r.remap);
+      hipLaunchKernelGGL((ens_z_kernel<LIM, RS, UNIF, P2>), grd, blk, r.lds_stage, s, r.d_mem, st, r.d_sw, r.nsw,
+                         r.remap);
     }
     return;
   }
   if (r.comp)
-    hipLaunchKernelGGL((ens_stage_comp_kernel<LIM, RS, UNIF>), dim3(g), dim3(kEnsBlock), r.lds_stage, s, r.d_mem,
-                       st, r.d_sw, r.nsw, r.remap);
+    hipLaunchKernelGGL((ens_stage_comp_kernel<LIM, RS, UNIF, P2>), grd, blk, r.lds_stage, s, r.d_mem, st, r.d_sw,
+                       r.nsw, r.remap);
   else
-    hipLaunchKernelGGL((ens_stage_kernel<LIM, RS, UNIF>), dim3(g), dim3(kEnsBlock), r.lds_stage, s, r.d_mem, st,
-                       r.d_sw, r.nsw, r.remap);
+    hipLaunchKernelGGL((ens_stage_kernel<LIM, RS, UNIF, P2>), grd, blk, r.lds_stage, s, r.d_mem, st, r.d_sw, r.nsw,
+                       r.remap);
 }
 
 template <int LIM, int RS>
 static void launch_ens_stage_u(const EnsRun& r, const StageArgs* st, bool fin, hipStream_t s) {
-  if (r.unif) launch_ens_stage_t<LIM, RS, true>(r, st, fin, s);
-  else launch_ens_stage_t<LIM, RS, false>(r, st, fin, s);
+  if (r.unif && r.p2) launch_ens_stage_t<LIM, RS, true, true>(r, st, fin, s);
+  else if (r.unif) launch_ens_stage_t<LIM, RS, true, false>(r, st, fin, s);
+  else launch_ens_stage_t<LIM, RS, false, false>(r, st, fin, s);
 }
 
 template <int LIM>
@@ -469,7 +479,10 @@ static void launch_ens_closure(const EnsRun& r, int sel, hipStream_t s) {
   const size_t lds = 3 * (4 + 2) * (64 + 2) * sizeof(R);
   const dim3 blk(64, 4, 1);
   if (r.visc) {
-    if (r.unif)
+    if (r.unif && r.p2)
+      hipLaunchKernelGGL((ens_closure_kernel<true, true, R, true>), dim3(g), blk, lds, s, r.d_mem, r.par, r.d_cw,
+                         r.ncw, sel, r.remap);
+    else if (r.unif)
       hipLaunchKernelGGL((ens_closure_kernel<true, true, R>), dim3(g), blk, lds, s, r.d_mem, r.par, r.d_cw, r.ncw,
                          sel, r.remap);
     else
@@ -533,7 +546,11 @@ hipError_t ens_advance(const mfvm_ens_member* m, int n, const mfvm_ns_params* p,
   r.visc = r.par.visc != 0;
   r.cfl = cfl;
   r.unif = true;
-  for (int i = 0; i < n; ++i) r.unif = r.unif && m[i].grid.uniform;
+  r.p2 = true;
+  for (int i = 0; i < n; ++i) {
+    r.unif = r.unif && m[i].grid.uniform;
+    r.p2 = r.p2 && pow2_geo(make_geo(&m[i].grid));
+  }
   r.d_mem = reinterpret_cast<EnsMem*>(base + L.off_mem);
   r.d_st = reinterpret_cast<StageArgs*>(base + L.off_st);
   r.d_sw = reinterpret_cast<EnsWork*>(base + L.off_sw);

@@ -41,13 +41,7 @@ __global__ void ns_closure_p_kernel(Layout lay, NSPar p, const R* __restrict__ U
   C[c] = closure_p(p, rho, v[0], v[1], v[2], U[c + 4 * fs]);
 }
 
-// P2 (every kernel below): every cell length and the volume are one power of
-// two (host-checked, pow2_geo), so the exact reciprocal multiplies are known
-// to apply at compile time instead of being tested at each use.
-MFVM_DEV void set_pow2(Geo& g) {
-  g.lpow2[0] = g.lpow2[1] = g.lpow2[2] = 1;
-  g.vpow2 = 1;
-}
+// P2 (every kernel below): uniform power-of-two geometry (set_pow2).
 
 template <bool UNIF, class R, bool P2>
 __global__ __launch_bounds__(256) void ns_closure_visc_kernel(Layout lay, Geo geo, NSPar p,
