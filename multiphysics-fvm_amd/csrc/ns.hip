@@ -71,10 +71,18 @@ __global__ __launch_bounds__(256, 2) void ns_stage_comp_kernel(StageArgs a) {
 // of 110 VGPRs, five waves per SIMD instead of four).  Other tile shapes and
 // larger arrays take FIXED = false.
 //
-// P2 (with FIXED): uniform power-of-two geometry, as for the kernels above.
-template <int LIM, int RS, bool UNIF, class R, bool FIXED, bool P2 = false>
+// SPEC (this kernel and the x3 kernels below): 1 -- uniform power-of-two
+// geometry, as for the kernels above; 2 -- that and a viscous flow (mu != 0,
+// host-checked), so the viscous terms are compiled in without a flag test
+// (spec_args).  0: both read at run time.
+MFVM_DEV void spec_args(StageArgs& a, int spec) {
+  if (spec >= 1) set_pow2(a.geo);
+  if (spec == 2) a.p.visc = 1;
+}
+
+template <int LIM, int RS, bool UNIF, class R, bool FIXED, int SPEC = 0>
 __global__ __launch_bounds__(256, FIXED && sizeof(R) == 8 ? 3 : 1) void ns_xy_kernel(StageArgs a) {
-  if constexpr (P2) set_pow2(a.geo);
+  spec_args(a, SPEC);
   TileCtx t = grid_tile(a);
   if constexpr (FIXED) {
     t.bx = 64;
@@ -86,17 +94,17 @@ __global__ __launch_bounds__(256, FIXED && sizeof(R) == 8 ? 3 : 1) void ns_xy_ke
 }
 
 // (Buffer-resource reads measure the same here as pointer reads.)
-template <int LIM, int RS, bool UNIF, bool P2>
+template <int LIM, int RS, bool UNIF, int SPEC>
 __global__ __launch_bounds__(256) void ns_z_kernel(StageArgs a) {
-  if constexpr (P2) set_pow2(a.geo);
+  spec_args(a, SPEC);
   ns_stage_body<LIM, RS, UNIF, double, false, true>(a, grid_tile(a), ptr_src<double>(a));
 }
 
 // FIN: the final stage (float64 accumulation and CFL partials) at two waves
 // per SIMD; the other stages leave that code out and fit three (144 VGPRs).
-template <int LIM, int RS, bool UNIF, bool FIN, bool P2>
+template <int LIM, int RS, bool UNIF, bool FIN, int SPEC>
 __global__ __launch_bounds__(256, FIN ? 2 : 3) void ns_z_comp_kernel(StageArgs a) {
-  if constexpr (P2) set_pow2(a.geo);
+  spec_args(a, SPEC);
   ns_stage_body<LIM, RS, UNIF, float, true, true, PtrSrc<float>, FIN ? 1 : 0>(a, grid_tile(a), ptr_src<float>(a));
 }
 
@@ -169,6 +177,19 @@ hipError_t launch_closure(const mfvm_grid* g, const mfvm_ns_params* p, const voi
   return hipGetLastError();
 }
 
+template <int LIM, int RS, bool UNIF, class R, bool FIXED, int SPEC>
+static void xy_launch(dim3 g, dim3 b, size_t lds, hipStream_t s, const StageArgs& a) {
+  hipLaunchKernelGGL((ns_xy_kernel<LIM, RS, UNIF, R, FIXED, SPEC>), g, b, lds, s, a);
+}
+
+// The x3 pass: float64, or the compensated float32 form (final stage or not).
+template <int LIM, int RS, bool UNIF, int SPEC>
+static void z_launch(bool comp, bool fin, dim3 g, dim3 b, size_t red, hipStream_t s, const StageArgs& a) {
+  if (comp && fin) hipLaunchKernelGGL((ns_z_comp_kernel<LIM, RS, UNIF, true, SPEC>), g, b, red, s, a);
+  else if (comp) hipLaunchKernelGGL((ns_z_comp_kernel<LIM, RS, UNIF, false, SPEC>), g, b, red, s, a);
+  else hipLaunchKernelGGL((ns_z_kernel<LIM, RS, UNIF, SPEC>), g, b, red, s, a);
+}
+
 // Planes per workgroup of the split x1/x2 pass (measured best at 256^3: 8
 // against 16 and 32).
 constexpr int XY_KZ = 8;
@@ -190,27 +211,21 @@ static hipError_t launch_stage_t(const StageArgs& a, bool unif, dim3 grd, dim3 b
     const dim3 gx(grd.x, grd.y, (a.lay.n3 + ax.kz - 1) / ax.kz);
     const bool fixed = blk.x == 64 && blk.y == 4 && buf_src_ok(a.lay, sizeof(R));
     const bool p2 = pow2_geo(a.geo);
+    const int spec = unif && p2 ? (a.p.visc ? 2 : 1) : 0;
     const int tx = t_begin(MFVM_TK_STAGE, s);
-    if (unif && fixed && p2) hipLaunchKernelGGL((ns_xy_kernel<LIM, RS, true, R, true, true>), gx, blk, lds, s, ax);
-    else if (unif && fixed) hipLaunchKernelGGL((ns_xy_kernel<LIM, RS, true, R, true>), gx, blk, lds, s, ax);
-    else if (unif) hipLaunchKernelGGL((ns_xy_kernel<LIM, RS, true, R, false>), gx, blk, lds, s, ax);
-    else if (fixed) hipLaunchKernelGGL((ns_xy_kernel<LIM, RS, false, R, true>), gx, blk, lds, s, ax);
-    else hipLaunchKernelGGL((ns_xy_kernel<LIM, RS, false, R, false>), gx, blk, lds, s, ax);
+    if (unif && fixed && spec == 2) xy_launch<LIM, RS, true, R, true, 2>(gx, blk, lds, s, ax);
+    else if (unif && fixed && spec == 1) xy_launch<LIM, RS, true, R, true, 1>(gx, blk, lds, s, ax);
+    else if (unif && fixed) xy_launch<LIM, RS, true, R, true, 0>(gx, blk, lds, s, ax);
+    else if (unif) xy_launch<LIM, RS, true, R, false, 0>(gx, blk, lds, s, ax);
+    else if (fixed) xy_launch<LIM, RS, false, R, true, 0>(gx, blk, lds, s, ax);
+    else xy_launch<LIM, RS, false, R, false, 0>(gx, blk, lds, s, ax);
     t_end(tx, s);
     const int tz = t_begin(MFVM_TK_OTHER, s);
-    if (comp) {
-      const bool fin = a.comp >= MFVM_COMP_FINAL_SET;
-      if (unif && p2 && fin) hipLaunchKernelGGL((ns_z_comp_kernel<LIM, RS, true, true, true>), grd, blk, red, s, a);
-      else if (unif && p2) hipLaunchKernelGGL((ns_z_comp_kernel<LIM, RS, true, false, true>), grd, blk, red, s, a);
-      else if (unif && fin) hipLaunchKernelGGL((ns_z_comp_kernel<LIM, RS, true, true, false>), grd, blk, red, s, a);
-      else if (unif) hipLaunchKernelGGL((ns_z_comp_kernel<LIM, RS, true, false, false>), grd, blk, red, s, a);
-      else if (fin) hipLaunchKernelGGL((ns_z_comp_kernel<LIM, RS, false, true, false>), grd, blk, red, s, a);
-      else hipLaunchKernelGGL((ns_z_comp_kernel<LIM, RS, false, false, false>), grd, blk, red, s, a);
-    } else {
-      if (unif && p2) hipLaunchKernelGGL((ns_z_kernel<LIM, RS, true, true>), grd, blk, red, s, a);
-      else if (unif) hipLaunchKernelGGL((ns_z_kernel<LIM, RS, true, false>), grd, blk, red, s, a);
-      else hipLaunchKernelGGL((ns_z_kernel<LIM, RS, false, false>), grd, blk, red, s, a);
-    }
+    const bool fin = a.comp >= MFVM_COMP_FINAL_SET;
+    if (spec == 2) z_launch<LIM, RS, true, 2>(comp, fin, grd, blk, red, s, a);
+    else if (spec == 1) z_launch<LIM, RS, true, 1>(comp, fin, grd, blk, red, s, a);
+    else if (unif) z_launch<LIM, RS, true, 0>(comp, fin, grd, blk, red, s, a);
+    else z_launch<LIM, RS, false, 0>(comp, fin, grd, blk, red, s, a);
     t_end(tz, s);
     return hipGetLastError();
   }
