@@ -15,8 +15,11 @@
            MUSCL+MC, Rusanov, RK3, mu=1e-4, periodic, 500 steps; autotuned
            launch shape, graph replay; FP64 and FP32-compensated, against
            advancing the members one by one.
+  diagnostics  mfvm_diagnostics on the Rayleigh-Taylor 256^3 state (FP64
+           alone, FP64 against a reference state, FP32-compensated against it):
+           kernel time and HBM rate.
 
-Usage: python bench_configs.py [kh128|maxwell|heat|ensemble|all] [--steps N] [--no-cpu]
+Usage: python bench_configs.py [kh128|maxwell|heat|ensemble|diagnostics|all] [--steps N] [--no-cpu]
 """
 
 import argparse
@@ -277,6 +280,46 @@ def ensemble(args, torch):
     return out
 
 
+def diagnostics(args, torch):
+    """State diagnostics (mfvm_diagnostics) on the Rayleigh-Taylor 256^3 state:
+    kernel time per call and HBM rate over the compulsory bytes (each
+    interior value read once)."""
+    import bench
+    from mfvm import _lib
+    from mfvm.diagnostics import raw
+    from mfvm.grid.geometry import build_grid
+    from mfvm.physics import navier_stokes as ns
+    n = 256
+    cfg = bench.rt_config(n)
+    grid = build_grid("cartesian", cfg)
+    U = ns.allocate_u(cfg)
+    for k, f in zip(ns.VARS, bench.rt_fields(n)):
+        U[k][1:-1, 1:-1, 1:-1] = torch.from_numpy(f).cuda()
+    ref = ns.allocate_u(cfg)
+    ref.buf.copy_(U.buf)
+    Uc = ns.to_compensated(U)
+    lo = Uc.scratch("lo", tuple(Uc.buf.shape))
+    cells = n ** 3
+    calls = args.steps or 50
+    cases = [("f64_pressure", U, dict(gamma=1.4), 40),
+             ("f64_vs_ref", U, dict(ref=ref, gamma=1.4), 80),
+             ("f32_compensated_vs_ref", Uc, dict(ref=ref, gamma=1.4, lo=lo), 80)]
+    out = {"config": "diagnostics_rt256", "cells": cells, "calls": calls}
+    for name, X, kw, bpc in cases:
+        raw(X, grid, **kw)
+        torch.cuda.synchronize()
+        _lib.timing(1)
+        ms = _time(lambda: [raw(X, grid, **kw) for _ in range(calls)], torch)
+        kms, kcnt = _lib.timing(2)
+        _lib.timing(0)
+        k_ms = kms[_lib.TK_OTHER] / max(kcnt[_lib.TK_OTHER], 1)
+        out[name] = {"bytes_per_cell": bpc, "kernel_ms": round(k_ms, 4),
+                     "call_ms": round(ms / calls, 4),
+                     "GB_per_s": round(bpc * cells / (k_ms * 1e-3) / 1e9, 1),
+                     "hbm_frac": round(bpc * cells / (k_ms * 1e-3) / 8e12, 3)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("which", nargs="?", default="all")
@@ -284,7 +327,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     import torch
-    todo = ["kh128", "maxwell", "heat", "ensemble"] if args.which == "all" else [args.which]
+    todo = ["kh128", "maxwell", "heat", "ensemble", "diagnostics"] if args.which == "all" else [args.which]
     for name in todo:
         print(json.dumps(globals()[name](args, torch)), flush=True)
 

@@ -458,6 +458,29 @@ int mfvm_maxwell_advance(const mfvm_grid* g, const mfvm_maxwell_params* p,
                          const mfvm_ns_work* w, int32_t integrator, double cfl,
                          int32_t nsteps, void* stream);
 
+/* ---- diagnostics ---------------------------------------------------------------
+ * No counterpart in the reference, which has no run-time monitor: conserved
+ * totals, extrema, a positivity monitor and errors against a second state,
+ * reduced in float64 over the interior cells of one ghosted SoA state, for
+ * any physics (nvar 1..6).  The value of a cell is double(U) (+ double(lo)
+ * for a compensated FP32 state).  out (device, MFVM_DIAG_NOUT(nvar) doubles):
+ *   out[q]             sum over cells of U_q * V (the conserved totals)
+ *   out[nvar + q]      min U_q                 out[2*nvar + q]  max U_q
+ *   out[3*nvar + q]    sum |U_q - ref_q| * V / sum V (volume-weighted L1)
+ *   out[4*nvar + q]    max |U_q - ref_q|  (both 0 when ref is NULL)
+ *   out[5*nvar]        sum V
+ *   out[5*nvar + 1]    gamma > 0 (Navier-Stokes, nvar 5): min of the ideal-gas
+ *                      pressure of navier_stokes.py:94-100; else 0
+ *   out[5*nvar + 2]    gamma > 0: number of cells with !(rho > 0) or
+ *                      !(P > 0) (NaN counts); else 0
+ * ref: NULL or a float64 ghosted state of the same shape.  partials:
+ * mfvm_diag_partials_size() doubles of scratch.  Min and max propagate NaN.
+ * The reduction order is fixed, so equal inputs give equal outputs. */
+#define MFVM_DIAG_NOUT(nvar) (5 * (nvar) + 3)
+int64_t mfvm_diag_partials_size(const mfvm_grid* g, int32_t nvar);
+int mfvm_diagnostics(const mfvm_grid* g, int32_t nvar, int32_t dtype, const void* U, const void* lo,
+                     const void* ref, double gamma, double* partials, double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

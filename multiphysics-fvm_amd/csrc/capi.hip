@@ -42,6 +42,10 @@ template <class R> hipError_t closure_apply(const mfvm_grid* g, double gamma, co
 template <class R> hipError_t closure_select(const mfvm_grid* g, const void* T, void* out, const int* flag,
                                              int* count, hipStream_t s);
 hipError_t fill_int(int* p, long long n, int v, hipStream_t s);
+// Defined in diag.hip.
+long long diag_nblocks(const mfvm_grid* g);
+hipError_t diag_launch(const mfvm_grid* g, int nvar, int dtype, const void* U, const void* lo, const void* ref,
+                       double gamma, double* partials, double* out, hipStream_t s);
 // Defined in ensemble.hip.
 int ens_nstages(int integrator);
 long long ens_member_tiles(const mfvm_ens_member* m);
@@ -932,6 +936,34 @@ int mfvm_maxwell_advance(const mfvm_grid* g, const mfvm_maxwell_params* p, const
   }
   if (nsteps > 0 && (rc = mfvm_maxwell_dt(g, p, cfl, w->dts + nsteps, stream))) return rc;
   return MFVM_OK;
+}
+
+// ---- diagnostics ------------------------------------------------------------------
+
+static int check_diag_size(const mfvm_grid* g, int32_t nvar) {
+  int rc = check_grid(g);
+  if (rc) return rc;
+  if (nvar < 1 || nvar > 6) return fail(MFVM_E_ARG, "diagnostics: nvar must be 1..6");
+  if ((long long)g->n[0] * g->n[1] * g->n[2] > 0x7fffffffLL)
+    return fail(MFVM_E_UNSUP, "diagnostics: more than 2^31-1 interior cells");
+  return MFVM_OK;
+}
+
+int64_t mfvm_diag_partials_size(const mfvm_grid* g, int32_t nvar) {
+  if (check_diag_size(g, nvar)) return -1;
+  return diag_nblocks(g) * MFVM_DIAG_NOUT(nvar);
+}
+
+int mfvm_diagnostics(const mfvm_grid* g, int32_t nvar, int32_t dtype, const void* U, const void* lo,
+                     const void* ref, double gamma, double* partials, double* out, void* stream) {
+  int rc = check_diag_size(g, nvar);
+  if (rc) return rc;
+  if (dtype != MFVM_F64 && dtype != MFVM_F32) return fail(MFVM_E_ARG, "diagnostics: dtype must be f64 or f32");
+  if (!U || !partials || !out) return fail(MFVM_E_ARG, "diagnostics: NULL buffer");
+  if (lo && dtype != MFVM_F32) return fail(MFVM_E_ARG, "diagnostics: a compensation array needs dtype f32");
+  if (gamma > 0.0 && nvar != 5) return fail(MFVM_E_ARG, "diagnostics: the pressure monitor needs the 5 NS fields");
+  return hip_check(diag_launch(g, nvar, dtype, U, lo, ref, gamma, partials, out, static_cast<hipStream_t>(stream)),
+                   "diagnostics");
 }
 
 }  // extern "C"

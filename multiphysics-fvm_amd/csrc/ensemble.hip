@@ -83,8 +83,8 @@ __global__ __launch_bounds__(kEnsBlock) void ens_bc_kernel(const EnsMem* __restr
   dst[var * e.lay.fs + e.lay.at(i, j, k)] = shell_value<R>(e.bca, var, src, i, j, k);
 }
 
-// P2 (these kernels): every member's geometry is uniform powers of two
-// (set_pow2).
+// P2 / SPEC (these kernels): every member's geometry is uniform powers of
+// two (set_pow2); SPEC 2 also a viscous flow (spec_args).
 template <bool VISC, bool UNIF, class R, bool P2 = false>
 __global__ __launch_bounds__(kEnsBlock) void ens_closure_kernel(const EnsMem* __restrict__ mem, NSPar p,
                                                                 const EnsWork* __restrict__ work,
@@ -104,14 +104,14 @@ __global__ __launch_bounds__(kEnsBlock) void ens_closure_kernel(const EnsMem* __
   }
 }
 
-template <int LIM, int RS, bool UNIF, class R, bool COMP, bool P2 = false>
+template <int LIM, int RS, bool UNIF, class R, bool COMP, int SPEC = 0>
 MFVM_DEV void ens_stage_loop(const EnsMem* __restrict__ mem, const StageArgs* __restrict__ st,
                              const EnsWork* __restrict__ work, long long nwork, int remap) {
   for (long long v = blockIdx.x; v < nwork; v += gridDim.x) {
     const EnsWork w = work[ens_item(v, nwork, remap != 0)];
     const EnsMem& e = mem[w.m];
     StageArgs a = st[w.m];
-    if constexpr (P2) set_pow2(a.geo);
+    spec_args(a, SPEC);
     TileCtx tc;
     tc.bx = e.bx;
     tc.by = e.by;
@@ -124,25 +124,25 @@ MFVM_DEV void ens_stage_loop(const EnsMem* __restrict__ mem, const StageArgs* __
   }
 }
 
-template <int LIM, int RS, bool UNIF, bool P2>
+template <int LIM, int RS, bool UNIF, int SPEC>
 __global__ __launch_bounds__(kEnsBlock) void ens_stage_kernel(const EnsMem* __restrict__ mem,
                                                               const StageArgs* __restrict__ st,
                                                               const EnsWork* __restrict__ work,
                                                               long long nwork, int remap) {
-  ens_stage_loop<LIM, RS, UNIF, double, false, P2>(mem, st, work, nwork, remap);
+  ens_stage_loop<LIM, RS, UNIF, double, false, SPEC>(mem, st, work, nwork, remap);
 }
 
-template <int LIM, int RS, bool UNIF, bool P2>
+template <int LIM, int RS, bool UNIF, int SPEC>
 __global__ __launch_bounds__(kEnsBlock, 2) void ens_stage_comp_kernel(const EnsMem* __restrict__ mem,
                                                                       const StageArgs* __restrict__ st,
                                                                       const EnsWork* __restrict__ work,
                                                                       long long nwork, int remap) {
-  ens_stage_loop<LIM, RS, UNIF, float, true, P2>(mem, st, work, nwork, remap);
+  ens_stage_loop<LIM, RS, UNIF, float, true, SPEC>(mem, st, work, nwork, remap);
 }
 
 // Split stage over the work list: the x1/x2 pass of every member's tiles,
 // then the x3 pass (as ns_xy_kernel / ns_z_kernel of the single-grid path).
-template <int LIM, int RS, bool UNIF, class R, bool P2>
+template <int LIM, int RS, bool UNIF, class R, int SPEC>
 __global__ __launch_bounds__(kEnsBlock) void ens_xy_kernel(const EnsMem* __restrict__ mem,
                                                            const StageArgs* __restrict__ st,
                                                            const EnsWork* __restrict__ work, long long nwork,
@@ -151,7 +151,7 @@ __global__ __launch_bounds__(kEnsBlock) void ens_xy_kernel(const EnsMem* __restr
     const EnsWork w = work[ens_item(v, nwork, remap != 0)];
     const EnsMem& e = mem[w.m];
     StageArgs a = st[w.m];
-    if constexpr (P2) set_pow2(a.geo);
+    spec_args(a, SPEC);
     TileCtx tc;
     tc.bx = e.bx;
     tc.by = e.by;
@@ -164,14 +164,14 @@ __global__ __launch_bounds__(kEnsBlock) void ens_xy_kernel(const EnsMem* __restr
   }
 }
 
-template <int LIM, int RS, bool UNIF, class R, bool COMP, int FIN = 2, bool P2 = false>
+template <int LIM, int RS, bool UNIF, class R, bool COMP, int FIN = 2, int SPEC = 0>
 MFVM_DEV void ens_z_loop(const EnsMem* __restrict__ mem, const StageArgs* __restrict__ st,
                          const EnsWork* __restrict__ work, long long nwork, int remap) {
   for (long long v = blockIdx.x; v < nwork; v += gridDim.x) {
     const EnsWork w = work[ens_item(v, nwork, remap != 0)];
     const EnsMem& e = mem[w.m];
     StageArgs a = st[w.m];
-    if constexpr (P2) set_pow2(a.geo);
+    spec_args(a, SPEC);
     TileCtx tc;
     tc.bx = e.bx;
     tc.by = e.by;
@@ -184,22 +184,22 @@ MFVM_DEV void ens_z_loop(const EnsMem* __restrict__ mem, const StageArgs* __rest
   }
 }
 
-template <int LIM, int RS, bool UNIF, bool P2>
+template <int LIM, int RS, bool UNIF, int SPEC>
 __global__ __launch_bounds__(kEnsBlock) void ens_z_kernel(const EnsMem* __restrict__ mem,
                                                           const StageArgs* __restrict__ st,
                                                           const EnsWork* __restrict__ work, long long nwork,
                                                           int remap) {
-  ens_z_loop<LIM, RS, UNIF, double, false, 2, P2>(mem, st, work, nwork, remap);
+  ens_z_loop<LIM, RS, UNIF, double, false, 2, SPEC>(mem, st, work, nwork, remap);
 }
 
 // FIN: the final stage (float64 accumulation, CFL partials) or not, as in
 // ns_z_comp_kernel.
-template <int LIM, int RS, bool UNIF, bool FIN, bool P2>
+template <int LIM, int RS, bool UNIF, bool FIN, int SPEC>
 __global__ __launch_bounds__(kEnsBlock, FIN ? 2 : 3) void ens_z_comp_kernel(const EnsMem* __restrict__ mem,
                                                                             const StageArgs* __restrict__ st,
                                                                             const EnsWork* __restrict__ work,
                                                                             long long nwork, int remap) {
-  ens_z_loop<LIM, RS, UNIF, float, true, FIN ? 1 : 0, P2>(mem, st, work, nwork, remap);
+  ens_z_loop<LIM, RS, UNIF, float, true, FIN ? 1 : 0, SPEC>(mem, st, work, nwork, remap);
 }
 
 template <bool UNIF, class R>
@@ -413,41 +413,42 @@ static unsigned grid_for(long long nitems, int persist_blocks) {
   return (unsigned)nitems;
 }
 
-template <int LIM, int RS, bool UNIF, bool P2>
+template <int LIM, int RS, bool UNIF, int SPEC>
 static void launch_ens_stage_t(const EnsRun& r, const StageArgs* st, bool fin, hipStream_t s) {
   const unsigned g = grid_for(r.nsw, r.persist_blocks);
   const dim3 grd(g), blk(kEnsBlock);
   if (r.split) {
     if (r.comp) {
-      hipLaunchKernelGGL((ens_xy_kernel<LIM, RS, UNIF, float, P2>), grd, blk, r.lds_stage, s, r.d_mem, st, r.d_sw,
+      hipLaunchKernelGGL((ens_xy_kernel<LIM, RS, UNIF, float, SPEC>), grd, blk, r.lds_stage, s, r.d_mem, st, r.d_sw,
                          r.nsw, r.remap);
       if (fin)
-        hipLaunchKernelGGL((ens_z_comp_kernel<LIM, RS, UNIF, true, P2>), grd, blk, r.lds_stage, s, r.d_mem, st,
+        hipLaunchKernelGGL((ens_z_comp_kernel<LIM, RS, UNIF, true, SPEC>), grd, blk, r.lds_stage, s, r.d_mem, st,
                            r.d_sw, r.nsw, r.remap);
       else
-        hipLaunchKernelGGL((ens_z_comp_kernel<LIM, RS, UNIF, false, P2>), grd, blk, r.lds_stage, s, r.d_mem, st,
+        hipLaunchKernelGGL((ens_z_comp_kernel<LIM, RS, UNIF, false, SPEC>), grd, blk, r.lds_stage, s, r.d_mem, st,
                            r.d_sw, r.nsw, r.remap);
     } else {
-      hipLaunchKernelGGL((ens_xy_kernel<LIM, RS, UNIF, double, P2>), grd, blk, r.lds_stage, s, r.d_mem, st, r.d_sw,
+      hipLaunchKernelGGL((ens_xy_kernel<LIM, RS, UNIF, double, SPEC>), grd, blk, r.lds_stage, s, r.d_mem, st, r.d_sw,
                          r.nsw, r.remap);
-      hipLaunchKernelGGL((ens_z_kernel<LIM, RS, UNIF, P2>), grd, blk, r.lds_stage, s, r.d_mem, st, r.d_sw, r.nsw,
+      hipLaunchKernelGGL((ens_z_kernel<LIM, RS, UNIF, SPEC>), grd, blk, r.lds_stage, s, r.d_mem, st, r.d_sw, r.nsw,
                          r.remap);
     }
     return;
   }
   if (r.comp)
-    hipLaunchKernelGGL((ens_stage_comp_kernel<LIM, RS, UNIF, P2>), grd, blk, r.lds_stage, s, r.d_mem, st, r.d_sw,
+    hipLaunchKernelGGL((ens_stage_comp_kernel<LIM, RS, UNIF, SPEC>), grd, blk, r.lds_stage, s, r.d_mem, st, r.d_sw,
                        r.nsw, r.remap);
   else
-    hipLaunchKernelGGL((ens_stage_kernel<LIM, RS, UNIF, P2>), grd, blk, r.lds_stage, s, r.d_mem, st, r.d_sw, r.nsw,
+    hipLaunchKernelGGL((ens_stage_kernel<LIM, RS, UNIF, SPEC>), grd, blk, r.lds_stage, s, r.d_mem, st, r.d_sw, r.nsw,
                        r.remap);
 }
 
 template <int LIM, int RS>
 static void launch_ens_stage_u(const EnsRun& r, const StageArgs* st, bool fin, hipStream_t s) {
-  if (r.unif && r.p2) launch_ens_stage_t<LIM, RS, true, true>(r, st, fin, s);
-  else if (r.unif) launch_ens_stage_t<LIM, RS, true, false>(r, st, fin, s);
-  else launch_ens_stage_t<LIM, RS, false, false>(r, st, fin, s);
+  if (r.unif && r.p2 && r.visc) launch_ens_stage_t<LIM, RS, true, 2>(r, st, fin, s);
+  else if (r.unif && r.p2) launch_ens_stage_t<LIM, RS, true, 1>(r, st, fin, s);
+  else if (r.unif) launch_ens_stage_t<LIM, RS, true, 0>(r, st, fin, s);
+  else launch_ens_stage_t<LIM, RS, false, 0>(r, st, fin, s);
 }
 
 template <int LIM>

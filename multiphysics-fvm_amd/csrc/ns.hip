@@ -71,14 +71,7 @@ __global__ __launch_bounds__(256, 2) void ns_stage_comp_kernel(StageArgs a) {
 // of 110 VGPRs, five waves per SIMD instead of four).  Other tile shapes and
 // larger arrays take FIXED = false.
 //
-// SPEC (this kernel and the x3 kernels below): 1 -- uniform power-of-two
-// geometry, as for the kernels above; 2 -- that and a viscous flow (mu != 0,
-// host-checked), so the viscous terms are compiled in without a flag test
-// (spec_args).  0: both read at run time.
-MFVM_DEV void spec_args(StageArgs& a, int spec) {
-  if (spec >= 1) set_pow2(a.geo);
-  if (spec == 2) a.p.visc = 1;
-}
+// SPEC (this kernel and the x3 kernels below): see spec_args.
 
 template <int LIM, int RS, bool UNIF, class R, bool FIXED, int SPEC = 0>
 __global__ __launch_bounds__(256, FIXED && sizeof(R) == 8 ? 3 : 1) void ns_xy_kernel(StageArgs a) {

@@ -316,6 +316,15 @@ struct BufSrc {
   MFVM_DEV R c(long long o, int f) const { return ld(C, o, f); }
 };
 
+// Stage kernels specialised by SPEC (host-checked): 1 -- uniform power-of-
+// two geometry (set_pow2); 2 -- that and a viscous flow (mu != 0), so the
+// viscous terms are compiled in without a flag test.  0: both read at run
+// time.  Called on the kernel's own copy of the arguments.
+MFVM_DEV void spec_args(StageArgs& a, int spec) {
+  if (spec >= 1) set_pow2(a.geo);
+  if (spec == 2) a.p.visc = 1;
+}
+
 template <class R>
 MFVM_DEV PtrSrc<R> ptr_src(const StageArgs& a) {
   return PtrSrc<R>{static_cast<const R*>(a.U), static_cast<const R*>(a.C), a.lay.fs};

@@ -8,7 +8,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(os.path.dirname(PKG), "csrc")
 INCLUDE = os.path.join(os.path.dirname(os.path.dirname(PKG)), "include")
 LIB = os.path.join(PKG, "lib", "libmfvm.so")
-SOURCES = ("capi.hip", "ns.hip", "bc.hip", "heat.hip", "maxwell.hip", "ensemble.hip", "closure.hip")
+SOURCES = ("capi.hip", "ns.hip", "bc.hip", "heat.hip", "maxwell.hip", "ensemble.hip", "closure.hip", "diag.hip")
 ARCH = os.environ.get("MFVM_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [

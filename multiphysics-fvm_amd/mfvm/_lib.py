@@ -167,6 +167,9 @@ SIGNATURES = [
     ("mfvm_maxwell_advance", ctypes.c_int, [_G, ctypes.POINTER(MaxwellParams),
                                             ctypes.POINTER(NSWork), ctypes.c_int32,
                                             ctypes.c_double, ctypes.c_int32, _P]),
+    ("mfvm_diag_partials_size", ctypes.c_int64, [_G, ctypes.c_int32]),
+    ("mfvm_diagnostics", ctypes.c_int, [_G, ctypes.c_int32, ctypes.c_int32, _P, _P, _P, ctypes.c_double,
+                                        _P, _P, _P]),
 ]
 
 

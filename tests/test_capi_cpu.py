@@ -154,3 +154,32 @@ def test_ensemble_argument_checks():
     o.order = 5
     p.dtype = _lib.F64
     assert _lib.lib.mfvm_ens_advance(*args(_lib.INT_RK3, 0, 1, nb, 0)) == _lib.E_ARG
+
+
+def test_diagnostics_argument_checks():
+    from mfvm import _lib
+    g = _lib.Grid()
+    g.n[:] = [64, 32, 16]
+    g.coord = 0
+    g.fac = 1          # never dereferenced: every call below is rejected first
+    g.fstride = 65
+    P = ctypes.c_void_p
+    assert _lib.lib.mfvm_diag_partials_size(ctypes.byref(g), 0) == -1
+    assert _lib.lib.mfvm_diag_partials_size(ctypes.byref(g), 7) == -1
+    # 32768 cells at 1024 per block -> 32 blocks of 5*5+3 slots
+    assert _lib.lib.mfvm_diag_partials_size(ctypes.byref(g), 5) == 32 * 28
+    big = _lib.Grid()
+    big.n[:] = [2048, 1024, 1024]
+    big.fac, big.fstride = 1, 2049
+    assert _lib.lib.mfvm_diag_partials_size(ctypes.byref(big), 1) == -1
+    assert "2^31" in _lib.lib.mfvm_last_error().decode()
+
+    def call(nvar, dtype, U, lo, ref, gamma, part, out):
+        return _lib.lib.mfvm_diagnostics(ctypes.byref(g), nvar, dtype, U, lo, ref, gamma, part, out, None)
+    assert call(5, _lib.F64, None, None, None, 0.0, P(8), P(8)) == _lib.E_ARG
+    assert "NULL" in _lib.lib.mfvm_last_error().decode()
+    assert call(5, 9, P(8), None, None, 0.0, P(8), P(8)) == _lib.E_ARG
+    assert call(5, _lib.F64, P(8), P(8), None, 0.0, P(8), P(8)) == _lib.E_ARG
+    assert "compensation" in _lib.lib.mfvm_last_error().decode()
+    assert call(6, _lib.F64, P(8), None, None, 1.4, P(8), P(8)) == _lib.E_ARG
+    assert "pressure" in _lib.lib.mfvm_last_error().decode()
