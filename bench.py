@@ -40,7 +40,7 @@ XY_BYTES_PER_CELL = 136.0
 # of this workload (FETCH_SIZE, WRITE_SIZE in KiB per dispatch) scaled by what
 # the same counters read for a 1 GiB device copy (bytes read = FETCH_SIZE *
 # 1024 * 2.0, written = WRITE_SIZE * 1024 * 1.0); None where not measured.
-MEASURED_TRAFFIC = {("ns_xy_kernel", 256): (807092 * 2.0 + 655360 * 1.0) * 1024,
+MEASURED_TRAFFIC = {("ns_xy_kernel", 256): (824524 * 2.0 + 655360 * 1.0) * 1024,
                     ("ns_stage_kernel", 256): (2458328 * 2.0 + 696341 * 1.0) * 1024}
 
 
