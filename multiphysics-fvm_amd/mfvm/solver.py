@@ -91,11 +91,27 @@ def advance(setup, U, grid, nsteps):
                            setup["time_config"]["CFL"], nsteps)
 
 
-def _report(setup, U, t, n, t_end):
+def _diagnostics(setup, U, grid):
+    from .diagnostics import diagnostics
+    from .physics import navier_stokes as ns
+    gamma = setup["physical_parameters"]["heat_capacity_ratio"] if U.names == ns.VARS else None
+    return diagnostics(U, grid, gamma=gamma)
+
+
+def _report(setup, U, t, n, t_end, grid=None, base=None):
     save = setup.get("save_config", {})
     saver, freq = save.get("saver"), save.get("output_frequency", 0)
     if freq and n % freq == 0:
         print(f"n = {n:6d} | t = {t:.2f}/{t_end:.2f}")
+        if base is not None:
+            # save_config["diagnostics"] (an extension): the change of every
+            # conserved total since the start, and the positivity monitor
+            d = _diagnostics(setup, U, grid)
+            drift = " ".join(f"{k}={d['total'][k] - base['total'][k]:+.3e}" for k in U.names)
+            line = f"           total drift: {drift}"
+            if d["nonpositive_cells"] is not None:
+                line += f" | min P {d['pressure_min']:.4e}, cells with rho<=0 or P<=0: {d['nonpositive_cells']}"
+            print(line)
         if saver is not None:
             saver(U, setup, t, n, save.get("directory"))
 
@@ -110,7 +126,10 @@ def run(setup):
     ``time_config["n_steps"]`` the run takes exactly that many steps with dt
     kept on the device (chunks between outputs, same dts and results as the
     step-by-step loop); with a ``closure`` entry (learned flux correction) it
-    advances to exactly ``t_end`` with :func:`mfvm.closure.advance`.
+    advances to exactly ``t_end`` with :func:`mfvm.closure.advance`; with
+    ``save_config["diagnostics"]`` each output line is followed by the drift
+    of the conserved totals since the start and, for Navier-Stokes, the
+    positivity monitor (:mod:`mfvm.diagnostics`).
     Returns U."""
     physics = setup["physics"]
     params = setup["physical_parameters"]
@@ -124,6 +143,7 @@ def run(setup):
         return _run_closure(setup, U, grid)
     save = setup.get("save_config", {})
     freq = save.get("output_frequency", 0)
+    base = _diagnostics(setup, U, grid) if save.get("diagnostics") else None
     if "n_steps" in tc:
         nsteps = int(tc["n_steps"])
         t, n = tc.get("t_start", 0.0), 0
@@ -133,7 +153,8 @@ def run(setup):
             dts = advance(setup, U, grid, chunk).cpu().numpy()
             last = n + chunk - 1
             if freq and last % freq == 0:
-                _report(setup, U, t + float(dts[:chunk - 1].sum()), last, tc.get("t_end", float("inf")))
+                _report(setup, U, t + float(dts[:chunk - 1].sum()), last, tc.get("t_end", float("inf")),
+                        grid, base)
             t += float(dts[:chunk].sum())
             n += chunk
         return U
@@ -142,7 +163,7 @@ def run(setup):
         dt = physics.compute_dt(U, params, grid, tc["CFL"])
         setup["time_integrator"](physics.compute_rhs, U, dt, params, grid, setup["space_schemes"],
                                  bcs, apply_bcs)
-        _report(setup, U, t, n, tc["t_end"])
+        _report(setup, U, t, n, tc["t_end"], grid, base)
         t += dt
         n += 1
     return U

@@ -42,6 +42,29 @@ def test_setup_run_matches_oracle():
         assert np.array_equal(U[k].cpu().numpy(), Uo[k]), k
 
 
+def test_setup_run_reports_diagnostics(capsys):
+    """save_config["diagnostics"]: every output line is followed by the drift
+    of the conserved totals and the positivity monitor; the run itself is
+    unchanged (bitwise equal to one without diagnostics)."""
+    from mfvm import solver
+    plain = solver.run(_small_kh(steps=5))
+    s = _small_kh(steps=5, freq=2)
+    s["save_config"]["saver"] = None
+    s["save_config"]["diagnostics"] = True
+    capsys.readouterr()
+    U = solver.run(s)
+    out = capsys.readouterr().out.splitlines()
+    torch.cuda.synchronize()
+    assert torch.equal(U.buf, plain.buf)
+    lines = [l for l in out if "total drift:" in l]
+    assert len(lines) == len([l for l in out if l.startswith("n = ")]) >= 2
+    for l in lines:
+        assert "mass_density=" in l and "energy_density=" in l
+        assert l.rstrip().endswith("cells with rho<=0 or P<=0: 0")
+        mass = float(l.split("mass_density=")[1].split()[0])
+        assert abs(mass) < 1e-6
+
+
 def test_snapshot_resume_is_exact(tmp_path):
     from mfvm import io, solver
     from mfvm.grid.geometry import build_grid
