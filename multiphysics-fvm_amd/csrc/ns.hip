@@ -41,7 +41,7 @@ __global__ void ns_closure_p_kernel(Layout lay, NSPar p, const R* __restrict__ U
   C[c] = closure_p(p, rho, v[0], v[1], v[2], U[c + 4 * fs]);
 }
 
-// P2 (every kernel below): uniform power-of-two geometry (set_pow2).
+// P2 (the closure kernel): uniform power-of-two geometry (set_pow2).
 
 template <bool UNIF, class R, bool P2>
 __global__ __launch_bounds__(256) void ns_closure_visc_kernel(Layout lay, Geo geo, NSPar p,
@@ -51,16 +51,17 @@ __global__ __launch_bounds__(256) void ns_closure_visc_kernel(Layout lay, Geo ge
   closure_march_body<true, UNIF, R>(lay, geo, p, U, C, kz, tile_id(remap != 0));
 }
 
-template <int LIM, int RS, bool UNIF, class R, bool P2>
+// The fused stage kernels take SPEC (spec_args) as the split ones do.
+template <int LIM, int RS, bool UNIF, class R, int SPEC>
 __global__ __launch_bounds__(256) void ns_stage_kernel(StageArgs a) {
-  if constexpr (P2) set_pow2(a.geo);
+  spec_args(a, SPEC);
   ns_stage_body<LIM, RS, UNIF, R, false, false>(a, grid_tile(a), ptr_src<R>(a));
 }
 
 // Compensated FP32 stage: half the registers of the FP64 kernel, two waves per SIMD.
-template <int LIM, int RS, bool UNIF, bool P2>
+template <int LIM, int RS, bool UNIF, int SPEC>
 __global__ __launch_bounds__(256, 2) void ns_stage_comp_kernel(StageArgs a) {
-  if constexpr (P2) set_pow2(a.geo);
+  spec_args(a, SPEC);
   ns_stage_body<LIM, RS, UNIF, float, true, false>(a, grid_tile(a), ptr_src<float>(a));
 }
 
@@ -224,19 +225,24 @@ static hipError_t launch_stage_t(const StageArgs& a, bool unif, dim3 grd, dim3 b
   }
   const int tk = t_begin(MFVM_TK_STAGE, s);
   const bool p2 = pow2_geo(a.geo);
+  const int spec = unif && p2 ? (a.p.visc ? 2 : 1) : 0;
   if (a.comp != MFVM_COMP_NONE) {
-    if (unif && p2)
-      hipLaunchKernelGGL((ns_stage_comp_kernel<LIM, RS, true, true>), grd, blk, lds, s, a);
+    if (spec == 2)
+      hipLaunchKernelGGL((ns_stage_comp_kernel<LIM, RS, true, 2>), grd, blk, lds, s, a);
+    else if (spec == 1)
+      hipLaunchKernelGGL((ns_stage_comp_kernel<LIM, RS, true, 1>), grd, blk, lds, s, a);
     else if (unif)
-      hipLaunchKernelGGL((ns_stage_comp_kernel<LIM, RS, true, false>), grd, blk, lds, s, a);
+      hipLaunchKernelGGL((ns_stage_comp_kernel<LIM, RS, true, 0>), grd, blk, lds, s, a);
     else
-      hipLaunchKernelGGL((ns_stage_comp_kernel<LIM, RS, false, false>), grd, blk, lds, s, a);
-  } else if (unif && p2)
-    hipLaunchKernelGGL((ns_stage_kernel<LIM, RS, true, R, true>), grd, blk, lds, s, a);
+      hipLaunchKernelGGL((ns_stage_comp_kernel<LIM, RS, false, 0>), grd, blk, lds, s, a);
+  } else if (spec == 2)
+    hipLaunchKernelGGL((ns_stage_kernel<LIM, RS, true, R, 2>), grd, blk, lds, s, a);
+  else if (spec == 1)
+    hipLaunchKernelGGL((ns_stage_kernel<LIM, RS, true, R, 1>), grd, blk, lds, s, a);
   else if (unif)
-    hipLaunchKernelGGL((ns_stage_kernel<LIM, RS, true, R, false>), grd, blk, lds, s, a);
+    hipLaunchKernelGGL((ns_stage_kernel<LIM, RS, true, R, 0>), grd, blk, lds, s, a);
   else
-    hipLaunchKernelGGL((ns_stage_kernel<LIM, RS, false, R, false>), grd, blk, lds, s, a);
+    hipLaunchKernelGGL((ns_stage_kernel<LIM, RS, false, R, 0>), grd, blk, lds, s, a);
   t_end(tk, s);
   return hipGetLastError();
 }
