@@ -220,7 +220,12 @@ def main():
         out["fp32_compensated"] = {"value": round(vc, 2), "unit": "Mcell-updates/s",
                                    "ms_per_step": round(msc / args.steps, 4),
                                    "speedup_vs_f64": round(vc / value, 3),
-                                   "bytes_per_cell_step": 240,
+                                   # state traffic per cell-step (FP32 words, 5 fields):
+                                   # stage 1 reads hi, writes d and the next state (60 B);
+                                   # stage 2 reads the state, hi and d, writes d and the
+                                   # next state (100 B); the last stage reads the state,
+                                   # hi, lo and d and writes hi and lo (120 B)
+                                   "bytes_per_cell_step": 280,
                                    "kernel_ms_per_step": {
                                        name: round(kmc[i] / args.steps, 4) for i, name in
                                        enumerate(("stage_xy" if split else "stage", "closure", "bc",

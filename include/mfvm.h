@@ -123,13 +123,14 @@ typedef struct mfvm_ns_params {  /* navier_stokes.py: params + space_schemes */
 #define MFVM_ACC_ADD 2    /* kacc = kacc + acc_coef*k */
 
 /* Compensated FP32 stages (dtype f32 with a compensation word per value):
- * U0 = hi + lo, stage states U0 + d with an FP32 increment d.
+ * U0 = hi + lo, FP32 stage states hi + d with an FP32 increment d.
  *   SET:        d = (c0*dt)*k               SSP: d = c0*(d + dt*k)
  *   FINAL_SET:  U0 += (c0*dt)*k             FINAL_SSP: U0 += c0*(d + dt*k)
  *   FINAL_ACC6: U0 += (dt/6)*(kacc + k)
- * Non-final stages write the next stage state hi + (lo + d) to snext; final
- * stages add the increment to hi + lo in FP64 and store the rounded sum and
- * its FP32 remainder in (out, lo).  No reference counterpart (the reference is
+ * Non-final stages write the next stage state hi + d to snext (an FP32
+ * value: lo, below half an ulp of hi, is left out of it); final stages add
+ * the increment to hi + lo in FP64 and store the rounded sum and its FP32
+ * remainder in (out, lo).  No reference counterpart (the reference is
  * FP64 only): this is the reduced-precision mode of the build. */
 #define MFVM_COMP_NONE 0
 #define MFVM_COMP_SET 1

@@ -938,9 +938,11 @@ MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl, const S& src)
           nu_max = nmax(nu_max, double(nu));
         }
       } else {
-        // Compensated update: U0 = hi + lo (two FP32 words), stage states are
-        // U0 + d with an FP32 increment d; the last stage adds its increment
-        // to hi + lo in FP64 and splits the sum back into (hi, lo).
+        // Compensated update: U0 = hi + lo (two FP32 words).  Stage states
+        // are the FP32 values hi + d with an FP32 increment d (lo, below half
+        // an ulp of hi, changes such a value by at most its own rounding, so
+        // the other stages do not read it); the last stage adds its
+        // increment to hi + lo in FP64 and splits the sum back into (hi, lo).
         R* lo = static_cast<R*>(a.lo);
         R* dv = static_cast<R*>(a.dinc);
         R* sn = static_cast<R*>(a.snext);
@@ -953,7 +955,7 @@ MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl, const S& src)
         for (int q = 0; q < 5; ++q) {
           const long long og = oc + q * fs, ii = oi + q * a.il.fs;
           hv[q] = U0[og];
-          lv[q] = lo[og];
+          lv[q] = fin ? lo[og] : R(0);
           dvv[q] = needd ? dv[ii] : R(0);
           kav[q] = needk ? ka[ii] : R(0);
         }
@@ -967,7 +969,7 @@ MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl, const S& src)
             if (a.comp == MFVM_COMP_SET) d = R(a.c0 * dt) * kq;
             else d = R(a.c0) * (dvv[q] + R(dt) * kq);
             dv[ii] = d;
-            sn[og] = h + (l + d);
+            sn[og] = h + d;
           } else {
             double inc;
             if (a.comp == MFVM_COMP_FINAL_SET) inc = (a.c0 * dt) * double(kq);
