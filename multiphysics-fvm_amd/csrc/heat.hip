@@ -116,6 +116,7 @@ MFVM_DEV void heat_dir(const HeatArgs& a, const R* U, int i, int j, int k, R& fs
 // ring is staged in LDS, then phi_1 and phi_2 at every position the tile's
 // x1 / x2 faces use, each once; the x3 faces read their column from memory.
 constexpr int HT_BX = 64, HT_BY = 4, HT_H = 3;
+static_assert(HT_BX == 64, "one wave per tile row");
 constexpr int HT_TX = HT_BX + 2 * HT_H, HT_TY = HT_BY + 2 * HT_H;  // T tile
 constexpr int HT_P1X = HT_BX + 4, HT_P2Y = HT_BY + 4;              // phi_1 / phi_2 tiles
 constexpr size_t ht_lds_elems() { return HT_TX * HT_TY + HT_P1X * HT_BY + HT_BX * HT_P2Y; }
@@ -128,7 +129,9 @@ __global__ __launch_bounds__(HT_BX * HT_BY) void heat_stage_kernel(HeatArgs a) {
   R* P2 = P1 + HT_P1X * HT_BY;             // [HT_P2Y][HT_BX], origin (i0, j0-2)
   const int n1 = a.lay.n1, n2 = a.lay.n2;
   const int i0 = 1 + blockIdx.x * HT_BX, j0 = 1 + blockIdx.y * HT_BY;
-  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * HT_BX + tx;
+  // A wave is one x2 row of the tile (HT_BX == 64): ty is wave-uniform, so
+  // the row's metric factors are read through the scalar cache.
+  const int tx = threadIdx.x, ty = __builtin_amdgcn_readfirstlane(threadIdx.y), tid = ty * HT_BX + tx;
   const int i = i0 + tx, j = j0 + ty;
   const int k = 1 + blockIdx.z;
   const R* U = static_cast<const R*>(a.U);
