@@ -17,13 +17,44 @@ GPU, in float64 over the interior cells, and reads back a few numbers:
 The reduction order is fixed, so equal states give equal results.
 """
 
-import ctypes
-
 import torch
 
 from . import _lib
 from .physics import _common as c
 from .state import State
+
+
+def _checked(U, grid, ref, gamma, lo):
+    """Validate the arguments the kernel cannot check (it takes plain
+    pointers); returns (grid descriptor, nvar, partials size)."""
+    if not isinstance(U, State):
+        raise TypeError("U must be an mfvm State; got " + type(U).__name__)
+    if U.device.type != "cuda":
+        raise RuntimeError("mfvm kernels need U on a HIP device")
+    g = c.require_grid(grid, U)
+    if ref is not None:
+        if not isinstance(ref, State) or ref.names != U.names or tuple(ref.n) != tuple(U.n):
+            raise ValueError("ref must be a State with the fields and shape of U")
+        if ref.dtype != torch.float64 or ref.device != U.device:
+            raise ValueError("ref must be a float64 state on U's device")
+    if lo is not None:
+        if (U.dtype != torch.float32 or lo.dtype != torch.float32 or tuple(lo.shape) != tuple(U.buf.shape)
+                or lo.device != U.device or not lo.is_contiguous()):
+            raise ValueError("lo must be a contiguous float32 array shaped like the float32 state U")
+    if gamma is not None and not float(gamma) > 0.0:
+        raise ValueError("gamma must be > 0")
+    nv = len(U.names)
+    n = int(_lib.lib.mfvm_diag_partials_size(g.ptr(), nv))
+    if n < 0:
+        _lib.check(_lib.E_ARG)
+    return g, nv, n
+
+
+def _call(U, g, nv, ref, gamma, lo, partials, out):
+    _lib.check(_lib.lib.mfvm_diagnostics(
+        g.ptr(), nv, c.dtype_code(U), c.ptr(U.buf), None if lo is None else c.ptr(lo),
+        None if ref is None else c.ptr(ref.buf), 0.0 if gamma is None else float(gamma),
+        c.ptr(partials), c.ptr(out), c.stream()))
 
 
 def diagnostics(U, grid, ref=None, gamma=None, compensated=False):
@@ -35,38 +66,15 @@ def diagnostics(U, grid, ref=None, gamma=None, compensated=False):
     compensated -- U is the float32 state of
                    ``navier_stokes.to_compensated``: each value is hi + lo.
     """
-    if not isinstance(U, State):
-        raise TypeError("U must be an mfvm State; got " + type(U).__name__)
-    if U.device.type != "cuda":
-        raise RuntimeError("mfvm kernels need U on a HIP device")
-    g = c.require_grid(grid, U)
-    nv = len(U.names)
-    dtype = c.dtype_code(U)
-    ref_ptr = None
-    if ref is not None:
-        if not isinstance(ref, State) or ref.names != U.names or tuple(ref.n) != tuple(U.n):
-            raise ValueError("ref must be a State with the fields and shape of U")
-        if ref.dtype != torch.float64 or ref.device != U.device:
-            raise ValueError("ref must be a float64 state on U's device")
-        ref_ptr = c.ptr(ref.buf)
-    lo_ptr = None
+    lo = None
     if compensated:
-        if U.dtype != torch.float32 or "lo" not in U._work:
+        if not isinstance(U, State) or U.dtype != torch.float32 or "lo" not in U._work:
             raise ValueError("compensated diagnostics need the float32 state of to_compensated")
-        lo_ptr = c.ptr(U.scratch("lo", tuple(U.buf.shape)))
-    g_val = 0.0
-    if gamma is not None:
-        g_val = float(gamma)
-        if not g_val > 0.0:
-            raise ValueError("gamma must be > 0")
-    n = int(_lib.lib.mfvm_diag_partials_size(g.ptr(), nv))
-    if n < 0:
-        _lib.check(_lib.E_ARG)
+        lo = U.scratch("lo", tuple(U.buf.shape))
+    g, nv, n = _checked(U, grid, ref, gamma, lo)
     nout = 5 * nv + 3
-    partials = U.scratch64("diag_partials", n)
     out = U.scratch64("diag_out", nout)
-    _lib.check(_lib.lib.mfvm_diagnostics(g.ptr(), nv, dtype, c.ptr(U.buf), lo_ptr, ref_ptr, g_val,
-                                         c.ptr(partials), c.ptr(out), c.stream()))
+    _call(U, g, nv, ref, gamma, lo, U.scratch64("diag_partials", n), out)
     h = out[:nout].cpu().tolist()
 
     def per(base):
@@ -84,22 +92,11 @@ def diagnostics(U, grid, ref=None, gamma=None, compensated=False):
     }
 
 
-def raw(U, grid, ref=None, gamma=None, lo=None, out=None, partials=None):
-    """The C-ABI call on explicit device buffers, no host read-back: returns
-    ``out`` (float64, 5*nvar+3 entries) for use inside a timed or captured
-    region."""
-    g = c.require_grid(grid, U)
-    nv = len(U.names)
-    nout = 5 * nv + 3
-    n = int(_lib.lib.mfvm_diag_partials_size(g.ptr(), nv))
-    if n < 0:
-        _lib.check(_lib.E_ARG)
-    if partials is None:
-        partials = U.scratch64("diag_partials", n)
-    if out is None:
-        out = U.scratch64("diag_out", nout)
-    _lib.check(_lib.lib.mfvm_diagnostics(
-        g.ptr(), nv, c.dtype_code(U), c.ptr(U.buf), None if lo is None else c.ptr(lo),
-        None if ref is None else c.ptr(ref.buf), 0.0 if gamma is None else float(gamma),
-        c.ptr(partials), c.ptr(out), c.stream()))
-    return out[:nout]
+def raw(U, grid, ref=None, gamma=None, lo=None):
+    """The C-ABI call with no host read-back (for timed regions):
+    returns the device array of 5*nvar+3 float64 results (the per-state
+    scratch, overwritten by the next call on U)."""
+    g, nv, n = _checked(U, grid, ref, gamma, lo)
+    out = U.scratch64("diag_out", 5 * nv + 3)
+    _call(U, g, nv, ref, gamma, lo, U.scratch64("diag_partials", n), out)
+    return out[:5 * nv + 3]

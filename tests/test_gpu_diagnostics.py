@@ -150,6 +150,17 @@ def test_argument_errors():
         diagnostics(U, grid, compensated=True)    # float64 state
     with pytest.raises(ValueError):
         diagnostics(U, grid, ref=_state(MX, n, torch.float32, 14))
+    # the no-read-back form checks what the kernel cannot (plain pointers)
+    from mfvm.diagnostics import raw
+    with pytest.raises(ValueError):
+        raw(U, grid, ref=_state(MX, n, torch.float32, 14))     # half the bytes of float64
+    V = _state(MX, n, torch.float32, 15)
+    with pytest.raises(ValueError):
+        raw(V, grid, lo=torch.zeros(3, device=V.device))        # too short
+    with pytest.raises(ValueError):
+        raw(U, grid, lo=torch.zeros_like(V.buf))                # a float64 state has no lo
+    out = raw(V, grid, lo=torch.zeros_like(V.buf))
+    assert out.shape == (5 * 6 + 3,) and out.dtype == torch.float64
 
 
 def test_rt256_totals_and_conservation_at_full_size():
