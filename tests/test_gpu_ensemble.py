@@ -140,3 +140,56 @@ def test_autotune_keeps_results():
         torch.cuda.synchronize()
         assert torch.equal(dts[i], ref)
         assert torch.equal(ens_states[i].buf, U.buf)
+
+
+@pytest.mark.parametrize("kind", ["pow2", "curvilinear"])
+@pytest.mark.parametrize("compensated", [False, True])
+def test_ensemble_geometry_specialisations_equal_single_runs(kind, compensated):
+    """Members whose geometry selects the other kernel forms: all uniform
+    power-of-two (the viscous power-of-two specialisation, as in the
+    benchmark ensemble), and a mix with cylindrical and spherical members (the
+    metric-factor form).  Each member equals its single-grid run bit for bit,
+    dt sequence included."""
+    from mfvm.ensemble import Ensemble
+    from mfvm.grid import boundary_conditions as bc
+    from mfvm.grid.geometry import build_grid
+    from mfvm.numerics import time_integrators as ti
+    ns, params, schemes, _, _, _ = _setup([])
+    if kind == "pow2":
+        members = [("cartesian", (64, 64, 1), (0, 0, 0), (1, 1, 1)),
+                   ("cartesian", (32, 128, 1), (0, 0, 0), (1, 1, 1)),
+                   ("cartesian", (16, 16, 16), (0, 0, 0), (1, 1, 1))]
+    else:
+        members = [("cartesian", (48, 40, 1), (0, 0, 0), (1, 1, 1)),
+                   ("cylindrical", (40, 24, 8), (0.5, 0.0, 0.0), (1.5, np.pi / 2, 1.0)),
+                   ("spherical", (24, 16, 12), (0.5, 0.3, 0.0), (1.5, np.pi - 0.3, np.pi))]
+    grids, fields, bcs = [], [], []
+    for i, (coord, n, lo, hi) in enumerate(members):
+        cfg = {"x1_min": lo[0], "x1_max": hi[0], "nx1": n[0], "x2_min": lo[1], "x2_max": hi[1],
+               "nx2": n[1], "x3_min": lo[2], "x3_max": hi[2], "nx3": n[2]}
+        grids.append((cfg, build_grid(coord, cfg)))
+        fields.append(_kh_fields(n, 40 + i))
+        if coord == "cartesian":
+            bcs.append({k: {s: bc.periodic for s in bc.SIDES} for k in ns.VARS})
+        else:
+            bcs.append({k: {s: bc.neumann for s in bc.SIDES} for k in ns.VARS})
+
+    def make(cfg, f):
+        U = _state(ns, cfg, f)
+        return ns.to_compensated(U) if compensated else U
+    steps = 7
+    ens_states = [make(cfg, f) for (cfg, _), f in zip(grids, fields)]
+    ens = Ensemble(ens_states, [g for _, g in grids], params, schemes, bcs, ti.rk3, compensated=compensated)
+    print(f"ensemble {kind} compensated={compensated}", flush=True)
+    dts = ens.advance(0.4, steps, {"graph": 4})
+    torch.cuda.synchronize()
+    for i, ((cfg, g), f) in enumerate(zip(grids, fields)):
+        U = make(cfg, f)
+        run = ns.advance_compensated if compensated else ns.advance
+        ref_dts = run(U, g, params, schemes, bcs[i], ti.rk3._mfvm_code, 0.4, steps)
+        torch.cuda.synchronize()
+        assert torch.equal(dts[i], ref_dts), (kind, i)
+        assert torch.equal(ens_states[i].buf, U.buf), (kind, i)
+        if compensated:
+            shp = tuple(U.buf.shape)
+            assert torch.equal(ens_states[i].scratch("lo", shp), U.scratch("lo", shp)), (kind, i)
