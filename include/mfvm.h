@@ -392,7 +392,8 @@ typedef struct mfvm_cnn {
 } mfvm_cnn;
 
 typedef struct mfvm_closure_work {
-  void* x;                      /* BF16 nx1*nx2*4 */
+  void* x;                      /* unused (the first layer forms its input features from the
+                                   state); kept for layout compatibility, may be NULL */
   void* a1;                     /* BF16 nx1*nx2*32 */
   void* a2;                     /* BF16 nx1*nx2*32 */
   float* y;                     /* FP32 nx1*nx2*8 */

@@ -27,6 +27,8 @@ hipError_t launch_dt_append(const mfvm_grid*, const mfvm_ns_params*, const doubl
 hipError_t cnn_conv(int mode, int cin, int cout, int batch, int n1, int n2, const void* x, const void* w,
                     const float* b, const void* mask, void* y, hipStream_t s);
 bool cnn_shape_ok(int mode, int cin, int cout);
+hipError_t cnn_conv_features(const mfvm_grid* g, double gamma, const void* U, const float* shift, const float* scale,
+                             const void* w, const float* b, void* y, hipStream_t s);
 hipError_t cnn_wgrad(int cin, int cout, int batch, int n1, int n2, const void* x, const void* gy, float* gw,
                      float* gb, hipStream_t s);
 hipError_t cnn_pack(int cin, int cout, const float* w, void* wf, void* wb, hipStream_t s);
@@ -678,9 +680,9 @@ static int closure_step(const mfvm_grid* g, const mfvm_ns_params* p, const mfvm_
     st.xy = w->xy;  // split stages where the caller provides the scratch (same results)
     if (e == hipSuccess) e = launch_stage<double>(g, p, &st, s);
     if (!net) continue;
+    // features of the stage input and the first layer in one launch
     if (e == hipSuccess)
-      e = cnn_features<double>(g, p->gamma, 1, 0, seq[q].in, net->in_shift, net->in_scale, cw->x, s);
-    if (e == hipSuccess) e = cnn_conv(MFVM_CNN_FWD_RELU, 4, 32, 1, n1, n2, cw->x, net->wf[0], net->b[0], nullptr, cw->a1, s);
+      e = cnn_conv_features(g, p->gamma, seq[q].in, net->in_shift, net->in_scale, net->wf[0], net->b[0], cw->a1, s);
     if (e == hipSuccess) e = cnn_conv(MFVM_CNN_FWD_RELU, 32, 32, 1, n1, n2, cw->a1, net->wf[1], net->b[1], nullptr, cw->a2, s);
     if (e == hipSuccess) e = cnn_conv(MFVM_CNN_FWD_LINEAR, 32, 8, 1, n1, n2, cw->a2, net->wf[2], net->b[2], nullptr, cw->y, s);
     if (e == hipSuccess)
@@ -713,7 +715,7 @@ int mfvm_closure_advance(const mfvm_grid* g, const mfvm_ns_params* p, const mfvm
     for (int l = 0; l < 3; ++l)
       if (!net->wf[l] || !net->b[l]) return fail(MFVM_E_ARG, "closure_advance: NULL network weights");
     if (!net->in_shift || !net->in_scale || !net->out_scale) return fail(MFVM_E_ARG, "closure_advance: NULL scales");
-    if (!cw->x || !cw->a1 || !cw->a2 || !cw->y || !cw->T) return fail(MFVM_E_ARG, "closure_advance: NULL work buffer");
+    if (!cw->a1 || !cw->a2 || !cw->y || !cw->T) return fail(MFVM_E_ARG, "closure_advance: NULL work buffer");
   }
   if (t_end != t_end) return fail(MFVM_E_ARG, "closure_advance: t_end is NaN");
   if (nsteps < 0 || graph < 0) return fail(MFVM_E_ARG, "nsteps and graph must be >= 0");

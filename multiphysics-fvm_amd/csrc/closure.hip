@@ -45,6 +45,33 @@ enum { CNN_FWD_RELU = 0, CNN_FWD_LINEAR = 1, CNN_BWD_DATA = 2 };
 
 MFVM_DEV int wrap(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
+// The 4 network input features of the cell at offset o of a 2D state
+// (rho, v1, v2, p from the conserved fields; p as the reference closure with
+// v3), shifted, scaled and rounded to BF16.
+template <class R>
+MFVM_DEV bf16x4 cnn_feat4(long long fs, double gm1, const R* u, long long o, const float* shift, const float* scale) {
+  const double rho = double(u[o]);
+  const double v1 = double(u[o + fs]) / rho, v2 = double(u[o + 2 * fs]) / rho;
+  const double v3 = double(u[o + 3 * fs]) / rho;
+  const double p = gm1 * (double(u[o + 4 * fs]) - 0.5 * rho * (v1 * v1 + v2 * v2 + v3 * v3));
+  const double q[4] = {rho, v1, v2, p};
+  bf16x4 h;
+#pragma unroll
+  for (int f = 0; f < 4; ++f) h[f] = (short)f2bf((float(q[f]) - shift[f]) * scale[f]);
+  return h;
+}
+
+// FEAT (first layer of the closure's inference): the input fragment of a
+// pixel is computed from the float64 state by cnn_feat4 instead of read from
+// a feature tensor -- the same values, one launch fewer.
+struct FeatSrc {
+  Layout lay;
+  const double* U;
+  double gm1;
+  const float* shift;
+  const float* scale;
+};
+
 // One 3x3 convolution (cross-correlation, periodic padding).
 //   FWD_RELU:   y = relu(conv(x, w) + b)            (BF16 out)
 //   FWD_LINEAR: y = conv(x, w) + b                  (FP32 out)
@@ -52,10 +79,12 @@ MFVM_DEV int wrap(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 //               transposed-flipped layout, x = output gradient)
 // w: [9][round16(COUT)][CIN] BF16 (rows >= COUT are zero).  CIN % 4 == 0,
 // n1 % 16 == 0.  One wave per 16-pixel row segment (grid-stride).
-template <int CIN, int COUT, int MODE>
+template <int CIN, int COUT, int MODE, bool FEAT = false>
 __global__ __launch_bounds__(256) void cnn_conv_kernel(int batch, int n1, int n2, const uint16_t* __restrict__ x,
                                                       const uint16_t* __restrict__ w, const float* __restrict__ bias,
-                                                      const uint16_t* __restrict__ mask, void* __restrict__ y) {
+                                                      const uint16_t* __restrict__ mask, void* __restrict__ y,
+                                                      FeatSrc fsrc) {
+  static_assert(!FEAT || (CIN == 4 && MODE == CNN_FWD_RELU), "features feed the first layer only");
   constexpr int COUTP = Round16<COUT>::v;
   constexpr int NCB = COUTP / 16;
   constexpr int KP = Round16<kTap * CIN>::v;
@@ -69,7 +98,7 @@ __global__ __launch_bounds__(256) void cnn_conv_kernel(int batch, int n1, int n2
     const long long row = seg / (n1 / 16);
     const int yy = (int)(row % n2);
     const long long b = row / n2;
-    const uint16_t* xb = x + b * (long long)n2 * n1 * CIN;
+    const uint16_t* xb = FEAT ? x : x + b * (long long)n2 * n1 * CIN;
     f32x4 acc[NCB];
 #pragma unroll
     for (int cb = 0; cb < NCB; ++cb) acc[cb] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -80,7 +109,10 @@ __global__ __launch_bounds__(256) void cnn_conv_kernel(int batch, int n1, int n2
       bf16x4 bv = {0, 0, 0, 0};
       if (tap < kTap) {
         const int px = wrap(xs + c + (tap % 3) - 1, n1), py = wrap(yy + (tap / 3) - 1, n2);
-        bv = *reinterpret_cast<const bf16x4*>(xb + ((long long)py * n1 + px) * CIN + ci0);
+        if constexpr (FEAT)
+          bv = cnn_feat4<double>(fsrc.lay.fs, fsrc.gm1, fsrc.U, fsrc.lay.at(px + 1, py + 1, 1), fsrc.shift, fsrc.scale);
+        else
+          bv = *reinterpret_cast<const bf16x4*>(xb + ((long long)py * n1 + px) * CIN + ci0);
       }
       const int tw = tap < kTap ? tap : kTap - 1;
 #pragma unroll
@@ -252,16 +284,7 @@ __global__ void cnn_features_kernel(Layout lay, double gm1, int batch, long long
     const int j = (int)((t / lay.n1) % lay.n2);
     const long long b = t / ((long long)lay.n1 * lay.n2);
     const R* u = U + b * bs;
-    const long long o = lay.at(i + 1, j + 1, 1);
-    const double rho = double(u[o]);
-    const double v1 = double(u[o + lay.fs]) / rho, v2 = double(u[o + 2 * lay.fs]) / rho;
-    const double v3 = double(u[o + 3 * lay.fs]) / rho;
-    const double p = gm1 * (double(u[o + 4 * lay.fs]) - 0.5 * rho * (v1 * v1 + v2 * v2 + v3 * v3));
-    const double q[4] = {rho, v1, v2, p};
-    bf16x4 h;
-#pragma unroll
-    for (int f = 0; f < 4; ++f) h[f] = (short)f2bf((float(q[f]) - shift[f]) * scale[f]);
-    *reinterpret_cast<bf16x4*>(x + t * 4) = h;
+    *reinterpret_cast<bf16x4*>(x + t * 4) = cnn_feat4<R>(lay.fs, gm1, u, lay.at(i + 1, j + 1, 1), shift, scale);
   }
 }
 
@@ -401,13 +424,13 @@ static hipError_t conv_t(int mode, int batch, int n1, int n2, const void* x, con
   const uint16_t* ms = static_cast<const uint16_t*>(mask);
   if (mode == CNN_FWD_RELU)
     hipLaunchKernelGGL((cnn_conv_kernel<CIN, COUT, CNN_FWD_RELU>), dim3(g), dim3(256), 0, s, batch, n1, n2, xs, ws, b,
-                       ms, y);
+                       ms, y, FeatSrc{});
   else if (mode == CNN_FWD_LINEAR)
     hipLaunchKernelGGL((cnn_conv_kernel<CIN, COUT, CNN_FWD_LINEAR>), dim3(g), dim3(256), 0, s, batch, n1, n2, xs, ws,
-                       b, ms, y);
+                       b, ms, y, FeatSrc{});
   else
     hipLaunchKernelGGL((cnn_conv_kernel<CIN, COUT, CNN_BWD_DATA>), dim3(g), dim3(256), 0, s, batch, n1, n2, xs, ws,
-                       b, ms, y);
+                       b, ms, y, FeatSrc{});
   return hipGetLastError();
 }
 
@@ -419,6 +442,17 @@ hipError_t cnn_conv(int mode, int cin, int cout, int batch, int n1, int n2, cons
   if (cin == 32 && cout == 8) return conv_t<32, 8>(mode, batch, n1, n2, x, w, b, mask, y, s);
   if (cin == 8 && cout == 32) return conv_t<8, 32>(mode, batch, n1, n2, x, w, b, mask, y, s);
   return hipErrorInvalidValue;
+}
+
+// First layer of the closure's inference on one float64 2D state: features
+// and the 4 -> 32 ReLU convolution in one launch (cnn_features + cnn_conv).
+hipError_t cnn_conv_features(const mfvm_grid* g, double gamma, const void* U, const float* shift, const float* scale,
+                             const void* w, const float* b, void* y, hipStream_t s) {
+  const FeatSrc f{make_layout(g->n[0], g->n[1], g->n[2]), static_cast<const double*>(U), gamma - 1.0, shift, scale};
+  const unsigned gr = grid_waves((long long)g->n[1] * (g->n[0] / 16));
+  hipLaunchKernelGGL((cnn_conv_kernel<4, 32, CNN_FWD_RELU, true>), dim3(gr), dim3(256), 0, s, 1, g->n[0], g->n[1],
+                     nullptr, static_cast<const uint16_t*>(w), b, nullptr, y, f);
+  return hipGetLastError();
 }
 
 bool cnn_shape_ok(int mode, int cin, int cout) {

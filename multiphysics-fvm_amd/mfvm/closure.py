@@ -192,7 +192,6 @@ class FluxCorrectionCNN:
 def _closure_work(U):
     n1, n2, _ = U.n
     w = _lib.ClosureWork()
-    w.x = _int16(U, "cnn_x", (n2, n1, 4)).data_ptr()
     w.a1 = _int16(U, "cnn_a1", (n2, n1, 32)).data_ptr()
     w.a2 = _int16(U, "cnn_a2", (n2, n1, 32)).data_ptr()
     w.y = _typed(U, "cnn_y", (n2, n1, 8), torch.float32).data_ptr()
