@@ -184,9 +184,11 @@ static void z_launch(bool comp, bool fin, dim3 g, dim3 b, size_t red, hipStream_
   else hipLaunchKernelGGL((ns_z_kernel<LIM, RS, UNIF, SPEC>), g, b, red, s, a);
 }
 
-// Planes per workgroup of the split x1/x2 pass (measured best at 256^3: 8
-// against 16 and 32).
-constexpr int XY_KZ = 8;
+// Planes per workgroup of the split x1/x2 pass, measured at 256^3.  float64:
+// 8 (8 and 4 time the same, 2 is 3% slower, 16 and 32 slower too); float32:
+// 2 (3% faster than 8, the same kernel being VALU-bound at five waves per
+// SIMD, where a shorter tail pays).
+constexpr int XY_KZ_F64 = 8, XY_KZ_F32 = 2;
 
 template <int LIM, int RS, class R>
 static hipError_t launch_stage_t(const StageArgs& a, bool unif, dim3 grd, dim3 blk, size_t lds, hipStream_t s) {
@@ -196,11 +198,12 @@ static hipError_t launch_stage_t(const StageArgs& a, bool unif, dim3 grd, dim3 b
     // update and CFL (timed as "other")
     const size_t red = 2 * sizeof(double) * ((blk.x * blk.y + 63) / 64);
     // The x1/x2 pass keeps nothing from one plane to the next, so its
-    // workgroups take short chunks of XY_KZ planes (more workgroups, a
+    // workgroups take short chunks of XY_KZ_* planes (more workgroups, a
     // shorter tail); the x3 march keeps the longer chunk, which amortises
     // its start-up planes.
     StageArgs ax = a;
-    ax.kz = a.kz < XY_KZ ? a.kz : XY_KZ;
+    const int xkz = sizeof(R) == sizeof(float) ? XY_KZ_F32 : XY_KZ_F64;
+    ax.kz = a.kz < xkz ? a.kz : xkz;
     ax.partials = nullptr;
     const dim3 gx(grd.x, grd.y, (a.lay.n3 + ax.kz - 1) / ax.kz);
     const bool fixed = blk.x == 64 && blk.y == 4 && buf_src_ok(a.lay, sizeof(R));
