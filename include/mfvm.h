@@ -167,7 +167,10 @@ typedef struct mfvm_heat_params {  /* heat.py: params + space_schemes */
   int32_t bc[1][6];
   double rho_cp;                /* mass_density * specific_heat_capacity */
   double conductivity;          /* thermal_conductivity */
-  double source;                /* volumetric_heat_source */
+  double source;                /* volumetric_heat_source (scalar form) */
+  const void* source_field;     /* array form (heat.py:294-297 broadcasts an array q): the
+                                   source of every interior cell, [nx3][nx2][nx1] in the
+                                   state's dtype; NULL: the scalar `source` everywhere */
 } mfvm_heat_params;
 
 typedef struct mfvm_maxwell_params {  /* maxwell.py: params + space_schemes */
@@ -177,9 +180,12 @@ typedef struct mfvm_maxwell_params {  /* maxwell.py: params + space_schemes */
   int32_t bc[6][6];
   double mu;                    /* magnetic_permeability */
   double ep;                    /* electric_permittivity */
-  double sigma;                 /* electrical_conductivity */
-  double rho;                   /* charge_density */
+  double sigma;                 /* electrical_conductivity (scalar form) */
+  double rho;                   /* charge_density (scalar form) */
   double chi_b, chi_d;          /* magnetic / electric cleaning coefficients */
+  const void* sigma_field;      /* array forms (maxwell.py:71,475 broadcast arrays): sigma and */
+  const void* rho_field;        /* rho of every interior cell, [nx3][nx2][nx1] in the state's
+                                   dtype; NULL: the scalar value everywhere */
 } mfvm_maxwell_params;
 
 typedef struct mfvm_ns_work {   /* scratch for whole steps (all physics) */

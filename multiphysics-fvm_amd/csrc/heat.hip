@@ -24,6 +24,7 @@ struct HeatArgs {
   double rcp;       // rho*cp
   double negk;      // -k
   double q;         // volumetric source
+  const void* qf;   // per-cell source (interior layout, R) or NULL: q everywhere
 };
 
 template <class R>
@@ -179,7 +180,7 @@ __global__ __launch_bounds__(HT_BX * HT_BY) void heat_stage_kernel(HeatArgs a) {
   heat_dir<2, LIM, R>(a, U, i, j, k, l3, h3);
   const R x = -(((((h1 - l1) + h2) - l2) + h3) - l3);
   R kr = a.geo.vpow2 ? x * R(a.geo.vinv) : x / met<R>(a.geo, MFVM_MET_V, i - 1, j - 1, k - 1);
-  kr = kr + R(a.q);
+  kr = kr + (a.qf ? static_cast<const R*>(a.qf)[a.il.at(i, j, k)] : R(a.q));
   const long long oc = a.lay.at(i, j, k);
   const R u = U[oc];
   stage_store<R>(a.io, 1, a.lay.fs, a.il.fs, oc, a.il.at(i, j, k), &u, &kr, nullptr);
@@ -224,6 +225,7 @@ hipError_t heat_launch_stage(const mfvm_grid* g, const mfvm_heat_params* p, cons
   a.inv_rcp = a.rcp_pow2 ? 1.0 / p->rho_cp : 0.0;
   a.negk = -p->conductivity;
   a.q = p->source;
+  a.qf = p->source_field;
   switch (p->limiter) {
     case MFVM_LIM_NONE: return heat_launch_t<MFVM_LIM_NONE, R>(a, s);
     case MFVM_LIM_MINMOD: return heat_launch_t<MFVM_LIM_MINMOD, R>(a, s);

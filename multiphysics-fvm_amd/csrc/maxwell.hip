@@ -31,6 +31,7 @@ struct MXArgs {
   const void* Rr;  // residuals [2][ghosted]
   StageIO io;
   void* xy;        // split stage: interior [6] in-plane flux differences
+  const void* sigf;  // per-cell conductivity (interior layout, R) or NULL: p.negsigma
 };
 
 template <class R>
@@ -65,8 +66,10 @@ MFVM_DEV R mx_div(const Layout& lay, const Geo& g, const R* F, int i, int j, int
   return div;
 }
 
+// rhof: per-cell charge density (interior layout) or NULL: p.rho everywhere.
 template <class R>
-__global__ void maxwell_res_kernel(Layout lay, Geo geo, MXPar p, const R* __restrict__ U, R* __restrict__ Rr) {
+__global__ void maxwell_res_kernel(Layout lay, Geo geo, MXPar p, const R* __restrict__ U, R* __restrict__ Rr,
+                                   const R* __restrict__ rhof) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   const int k = blockIdx.z;
@@ -79,7 +82,8 @@ __global__ void maxwell_res_kernel(Layout lay, Geo geo, MXPar p, const R* __rest
   const int jj = j == 0 ? 1 : (j == n2 + 1 ? n2 : j);
   const int kk = k == 0 ? 1 : (k == n3 + 1 ? n3 : k);
   const R rB = mx_div<R>(lay, geo, U, ii, jj, kk);
-  const R rD = mx_div<R>(lay, geo, U + 3 * lay.fs, ii, jj, kk) - R(p.rho);
+  const long long ic = (long long)(kk - 1) * lay.n2 * lay.n1 + (long long)(jj - 1) * lay.n1 + (ii - 1);
+  const R rD = mx_div<R>(lay, geo, U + 3 * lay.fs, ii, jj, kk) - (rhof ? rhof[ic] : R(p.rho));
   const long long o = lay.at(i, j, k);
   Rr[o] = rB;
   Rr[o + lay.fs] = rD;
@@ -217,7 +221,9 @@ MFVM_DEV void mx_finish(const MXArgs& a, const R* __restrict__ Rr, int i, int j,
   R vol = a.geo.vpow2 ? R(0) : met<R>(a.geo, MFVM_MET_V, i - 1, j - 1, k - 1);
 #pragma unroll
   for (int q = 0; q < 6; ++q) {
-    const R sp = q < 3 ? R(0) : R(a.p.negsigma) * mx_E<R>(a.p, u[q]);
+    // J = -sigma E (maxwell.py:85-87); an array sigma is negated per cell
+    const R ns = a.sigf ? -static_cast<const R*>(a.sigf)[a.il.at(i, j, k)] : R(a.p.negsigma);
+    const R sp = q < 3 ? R(0) : ns * mx_E<R>(a.p, u[q]);
     const R src = (sg[q] + sp) + sc[q];
     const R x = -acc[q];
     kr[q] = (a.geo.vpow2 ? x * R(a.geo.vinv) : x / vol) + src;
@@ -566,7 +572,7 @@ hipError_t maxwell_launch_res(const mfvm_grid* g, const mfvm_maxwell_params* q, 
   const dim3 grd((g->n[0] + 2 + 63) / 64, (g->n[1] + 2 + 3) / 4, g->n[2] + 2);
   const int tk = t_begin(MFVM_TK_CLOSURE, s);
   hipLaunchKernelGGL(maxwell_res_kernel<R>, grd, blk, 0, s, lay, make_geo(g), make_mx(q),
-                     static_cast<const R*>(U), static_cast<R*>(Rr));
+                     static_cast<const R*>(U), static_cast<R*>(Rr), static_cast<const R*>(q->rho_field));
   t_end(tk, s);
   return hipGetLastError();
 }
@@ -612,6 +618,7 @@ hipError_t maxwell_launch_stage(const mfvm_grid* g, const mfvm_maxwell_params* q
   a.U = U;
   a.Rr = Rr;
   a.io = io;
+  a.sigf = q->sigma_field;
 #define MX_RS(L)                                                                          \
   return q->riemann == MFVM_RS_HLL ? mx_launch_t<L, MFVM_RS_HLL, R>(a, s) : mx_launch_t<L, MFVM_RS_RUSANOV, R>(a, s);
   switch (q->limiter) {

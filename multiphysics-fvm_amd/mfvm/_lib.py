@@ -67,14 +67,16 @@ class NSCompWork(ctypes.Structure):
 class HeatParams(ctypes.Structure):
     _fields_ = [("dtype", ctypes.c_int32), ("limiter", ctypes.c_int32),
                 ("bc", (ctypes.c_int32 * 6) * 1), ("rho_cp", ctypes.c_double),
-                ("conductivity", ctypes.c_double), ("source", ctypes.c_double)]
+                ("conductivity", ctypes.c_double), ("source", ctypes.c_double),
+                ("source_field", ctypes.c_void_p)]
 
 
 class MaxwellParams(ctypes.Structure):
     _fields_ = [("dtype", ctypes.c_int32), ("limiter", ctypes.c_int32),
                 ("riemann", ctypes.c_int32), ("bc", (ctypes.c_int32 * 6) * 6),
                 ("mu", ctypes.c_double), ("ep", ctypes.c_double), ("sigma", ctypes.c_double),
-                ("rho", ctypes.c_double), ("chi_b", ctypes.c_double), ("chi_d", ctypes.c_double)]
+                ("rho", ctypes.c_double), ("chi_b", ctypes.c_double), ("chi_d", ctypes.c_double),
+                ("sigma_field", ctypes.c_void_p), ("rho_field", ctypes.c_void_p)]
 
 
 class NSWork(ctypes.Structure):

@@ -1,7 +1,9 @@
 """Scheme resolution shared by the physics modules."""
 
 import ctypes
+import numbers
 
+import numpy as np
 import torch
 
 from .. import _lib
@@ -62,6 +64,27 @@ def bc_table(names, bcs):
     rows = bc_codes(names, bcs)
     flat = (ctypes.c_int32 * (6 * len(names)))(*[c for r in rows for c in r])
     return rows, flat
+
+
+def cell_param(U, value):
+    """A physical parameter that the reference broadcasts against interior
+    arrays: a real number -> (float, None); an array (NumPy, torch or nested
+    sequence) broadcastable to (nx1, nx2, nx3) -> (0.0, device tensor of the
+    value at every interior cell, [nx3][nx2][nx1] in U's dtype).  A 0-d array
+    counts as a number."""
+    if isinstance(value, numbers.Real):
+        return float(value), None
+    t = value if isinstance(value, torch.Tensor) else torch.as_tensor(np.asarray(value, dtype=np.float64))
+    if t.dim() == 0:
+        return float(t), None
+    n1, n2, n3 = U.n
+    try:
+        full = torch.broadcast_to(t, (n1, n2, n3))
+    except RuntimeError as e:
+        raise ValueError(f"parameter of shape {tuple(t.shape)} does not broadcast to the "
+                         f"interior shape {(n1, n2, n3)}") from e
+    field = full.permute(2, 1, 0).to(device=U.device, dtype=U.dtype).contiguous()
+    return 0.0, field
 
 
 def ptr(t):

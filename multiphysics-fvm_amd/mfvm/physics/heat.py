@@ -1,7 +1,6 @@
 """Heat equation (drop-in for src/physics/heat.py), computed in libmfvm.so."""
 
 import ctypes
-import numbers
 
 import torch
 
@@ -26,10 +25,11 @@ def make_params(U, params, space_schemes, bcs=None):
         raise NotImplementedError(f"diffusive solver {ds!r} has no fused kernel")
     p.rho_cp = float(params["mass_density"]) * float(params["specific_heat_capacity"])
     p.conductivity = float(params["thermal_conductivity"])
-    q = params["volumetric_heat_source"]
-    if not isinstance(q, numbers.Real):
-        raise NotImplementedError("volumetric_heat_source must be a scalar")
-    p.source = float(q)
+    # heat.py:294-297: q is a number or an array broadcast over the interior
+    p.source, field = c.cell_param(U, params["volumetric_heat_source"])
+    if field is not None:
+        p.source_field = field.data_ptr()
+        p._keep = field  # alive as long as the descriptor
     if bcs is not None:
         rows, _ = c.bc_table(VARS, bcs)
         p.bc[0][:] = rows[0]

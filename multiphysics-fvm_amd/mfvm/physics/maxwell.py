@@ -19,6 +19,11 @@ def allocate_u(grid_config, dtype=torch.float64, device="cuda"):
 
 
 def make_params(U, params, space_schemes=None, bcs=None):
+    """C descriptor from the reference dicts.  Keys are read as the reference
+    reads them (a missing one raises KeyError): the closure's permittivity
+    and conductivity always (maxwell.py:70-71, also on the dt path), the
+    charge density where the right-hand side needs it (:475).  Conductivity
+    and charge density may be numbers or arrays broadcast over the interior."""
     p = _lib.MaxwellParams()
     p.dtype = c.dtype_code(U)
     if space_schemes is not None:
@@ -26,8 +31,17 @@ def make_params(U, params, space_schemes=None, bcs=None):
         p.riemann = c.riemann_code(space_schemes)
     p.mu = float(params["magnetic_permeability"])
     p.ep = float(params["electric_permittivity"])
-    p.sigma = float(params.get("electrical_conductivity", 0.0))
-    p.rho = float(params.get("charge_density", 0.0))
+    keep = []
+    p.sigma, field = c.cell_param(U, params["electrical_conductivity"])
+    if field is not None:
+        p.sigma_field = field.data_ptr()
+        keep.append(field)
+    if space_schemes is not None:
+        p.rho, field = c.cell_param(U, params["charge_density"])
+        if field is not None:
+            p.rho_field = field.data_ptr()
+            keep.append(field)
+    p._keep = keep  # alive as long as the descriptor
     p.chi_b = float(params["magnetic_cleaning_coeff"])
     p.chi_d = float(params["electric_cleaning_coeff"])
     if bcs is not None:

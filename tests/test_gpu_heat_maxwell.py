@@ -290,3 +290,81 @@ def test_rhs_on_every_coordinate_system(physics, coord):
     r = gm.compute_rhs(U, build_grid(coord, gc), params, _schemes("muscl", "van_leer", rs))
     for k in gm.VARS:
         assert same(r[k].cpu().numpy(), ro[k]), k
+
+
+def _heated_disc(n, dtype):
+    """The heated-disc cylindrical state (r in [0, 1], T = 2 for r < 0.25)."""
+    from mfvm.grid.geometry import build_grid
+    from mfvm.physics import heat
+    from mfvm.setups import heated_disc_cylindrical as hd
+    s = hd.get_setup(n)
+    grid = build_grid("cylindrical", s["grid_config"])
+    U = heat.allocate_u(s["grid_config"], dtype=dtype)
+    r = (np.arange(n[0]) + 0.5) / n[0]
+    T0 = np.where(r < 0.25, 2.0, 1.0)[:, None, None] * np.ones(n)
+    U["internal_energy_density"][1:-1, 1:-1, 1:-1] = torch.from_numpy(T0).to(dtype).cuda()
+    return s, grid, U, T0
+
+
+def test_heat_fp32_cylindrical_full_size_near_fp64():
+    """The FP32 leg of the cylindrical heat configuration (256 x 128 x 128,
+    300 Euler steps, heat.py:354-473): against the FP64 run of the same
+    steps, L1 <= 2e-5 and L-inf <= 2e-4 for a field of order 1 (FP32 carries
+    about 7 digits and every step rounds the update once)."""
+    from mfvm import _lib
+    from mfvm.physics import heat
+    n = (256, 128, 128)
+    out = {}
+    for dtype in (torch.float64, torch.float32):
+        s, grid, U, T0 = _heated_disc(n, dtype)
+        print(f"heat fp32 leg: {dtype} 300 steps", flush=True)
+        dts = heat.advance(U, grid, s["physical_parameters"], s["space_schemes"], s["boundary_conditions"],
+                           _lib.INT_EULER, 0.4, 300)
+        torch.cuda.synchronize()
+        assert bool(torch.isfinite(dts).all())
+        out[dtype] = U.buf[:, 1:-1, 1:-1, 1:-1].to(torch.float64)
+    moved = (out[torch.float64] - out[torch.float64].new_tensor(T0.transpose(2, 1, 0)[None])).abs().max()
+    d = (out[torch.float32] - out[torch.float64]).abs()
+    l1, linf = float(d.mean()), float(d.max())
+    print(f"heat fp32 vs fp64 (300 steps): L1 {l1:.3e} Linf {linf:.3e}; "
+          f"largest change of the field {float(moved):.3e}", flush=True)
+    assert float(moved) > 1e-2  # the run did diffuse the disc
+    assert l1 <= 2e-5 and linf <= 2e-4, (l1, linf)
+
+
+def test_heat_fp32_reduced_size_near_oracle():
+    """FP32 heat against the oracle's FP64 run on a reduced grid (64 x 32 x
+    16, 50 Euler steps): L1 <= 1e-5, L-inf <= 1e-4; the FP64 run of the same
+    steps equals the oracle bitwise."""
+    from mfvm import _lib
+    from mfvm.physics import heat
+    from oracle import heat as oht
+    from oracle import numerics as on
+    from oracle import solver as osv
+    from oracle import grid as og
+    n = (64, 32, 16)
+    s, _, _, T0 = _heated_disc(n, torch.float64)
+    Uo = oht.allocate_u(s["grid_config"])
+    Uo["internal_energy_density"][1:-1, 1:-1, 1:-1] = T0
+    sides = {"left": og.neumann, "right": og.neumann, "bottom": og.periodic, "top": og.periodic,
+             "front": og.neumann, "back": og.neumann}
+    bo = {"internal_energy_density": sides}
+    go = og.build_grid("cylindrical", s["grid_config"])
+    so = {"reconstructor": on.piecewise_constant, "slope_limiter": on.minmod, "riemann_solver": on.hll,
+          "diffusive_solver": on.centered}
+    dts_o = osv.advance(oht, Uo, go, s["physical_parameters"], so, on.euler_explicit, bo, 0.4, 50)
+    ref = Uo["internal_energy_density"][1:-1, 1:-1, 1:-1].transpose(2, 1, 0)
+    for dtype in (torch.float64, torch.float32):
+        s, grid, U, _ = _heated_disc(n, dtype)
+        print(f"heat reduced size: {dtype} 50 steps", flush=True)
+        dts = heat.advance(U, grid, s["physical_parameters"], s["space_schemes"], s["boundary_conditions"],
+                           _lib.INT_EULER, 0.4, 50)
+        torch.cuda.synchronize()
+        got = U.buf[0, 1:-1, 1:-1, 1:-1].to(torch.float64).cpu().numpy()
+        if dtype == torch.float64:
+            assert dts.cpu().numpy()[:50].tolist() == dts_o
+            assert same(got, ref)
+        else:
+            d = np.abs(got - ref)
+            print(f"heat fp32 vs oracle (50 steps): L1 {d.mean():.3e} Linf {d.max():.3e}", flush=True)
+            assert d.mean() <= 1e-5 and d.max() <= 1e-4, (d.mean(), d.max())
