@@ -249,6 +249,22 @@ int mfvm_ns_rhs(const mfvm_grid* g, const mfvm_ns_params* p, const void* U, void
 int mfvm_ns_dt(const mfvm_grid* g, const mfvm_ns_params* p, const void* U, double cfl,
                double* partials, double* dt_out, void* stream);
 
+/* Largest grid of mfvm_ns_dt_noalloc, in CFL partial slots (workgroups of the
+ * partials kernel, mfvm_ns_partials_size() / 2): 65536 slots = 1 MiB of device
+ * memory per GPU, e.g. 512^3 cells with the default tile needs 16384. */
+#define MFVM_NS_DT_NOALLOC_MAX_BLOCKS 65536
+
+/* mfvm_ns_dt without the partials argument: the CFL partials go to a fixed
+ * device array inside the library (MFVM_NS_DT_NOALLOC_MAX_BLOCKS slots per
+ * GPU, kept until the process ends), so the call allocates nothing.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * (code under HIP graph capture, serving loops, allocation-free harnesses).
+ * Grids above the maximum fail with MFVM_E_ARG.  Calls that overlap on one
+ * GPU (several streams or host threads at once) share the array and race;
+ * serialise them. */
+int mfvm_ns_dt_noalloc(const mfvm_grid* g, const mfvm_ns_params* p, const void* U, double cfl,
+                       double* dt_out, void* stream);
+
 /* One fused Runge-Kutta stage: flux divergence of st->U (closure st->C),
  * sources, and the update of st->update, optionally the CFL partials of the
  * result.  Replaces one stage block of src/numerics/time_integrators.py

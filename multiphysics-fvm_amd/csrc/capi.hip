@@ -1,4 +1,5 @@
 // C-ABI of libmfvm.so: argument checks, error reporting and stage sequencing.
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -295,6 +296,61 @@ int mfvm_ns_dt(const mfvm_grid* g, const mfvm_ns_params* p, const void* U, doubl
   return hip_check(p->dtype == MFVM_F64 ? launch_cfl<double>(g, p, U, nullptr, cfl, partials, dt_out, s)
                                         : launch_cfl<float>(g, p, U, nullptr, cfl, partials, dt_out, s),
                    "ns_dt");
+}
+
+}  // extern "C"
+
+namespace mfvm {
+// ---- allocation-free CFL step --------------------------------------------------
+// The CFL partials of mfvm_ns_dt_noalloc live in this file-scope device array
+// (one copy per GPU, held until the process ends).  Host code never passes the
+// array's name to a kernel: it takes the device address once per device with
+// hipGetSymbolAddress and keeps it in ns_dt_scratch_ptr[device].
+static __device__ __align__(256) double scratch[2 * MFVM_NS_DT_NOALLOC_MAX_BLOCKS];
+
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* ns_dt_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (looked up on the first
+// call on that device, with it current).
+static int scratch_address(double** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_check(e, "ns_dt_noalloc: hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices) return fail(MFVM_E_ARG, "ns_dt_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!ns_dt_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_check(e, "ns_dt_noalloc: hipGetSymbolAddress");
+    ns_dt_scratch_ptr[dev] = static_cast<double*>(ptr);
+  }
+  *out = ns_dt_scratch_ptr[dev];
+  return MFVM_OK;
+}
+
+}  // namespace mfvm
+
+extern "C" {
+
+int mfvm_ns_dt_noalloc(const mfvm_grid* g, const mfvm_ns_params* p, const void* U, double cfl, double* dt_out,
+                       void* stream) {
+  int rc = check_ns(g, p);
+  if (rc) return rc;
+  if (!U || !dt_out) return fail(MFVM_E_ARG, "ns_dt_noalloc: NULL buffer");
+  if (ns_nblocks(g, p) > MFVM_NS_DT_NOALLOC_MAX_BLOCKS)
+    return fail(MFVM_E_ARG, "ns_dt_noalloc: the grid needs " + std::to_string(ns_nblocks(g, p)) +
+                                " CFL partial slots, above the fixed maximum of " +
+                                std::to_string(MFVM_NS_DT_NOALLOC_MAX_BLOCKS) + " (use mfvm_ns_dt)");
+  double* partials = nullptr;
+  if ((rc = scratch_address(&partials))) return rc;
+  // Every workgroup of the partials kernel writes its own pair of slots and
+  // the finalize reads only those, so nothing needs clearing between calls.
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return hip_check(p->dtype == MFVM_F64 ? launch_cfl<double>(g, p, U, nullptr, cfl, partials, dt_out, s)
+                                        : launch_cfl<float>(g, p, U, nullptr, cfl, partials, dt_out, s),
+                   "ns_dt_noalloc");
 }
 
 // Stage sequence of one integrator call (time_integrators.py) for any

@@ -129,6 +129,7 @@ SIGNATURES = [
     ("mfvm_ns_closure", ctypes.c_int, [_G, _NP, _P, _P, _P]),
     ("mfvm_ns_rhs", ctypes.c_int, [_G, _NP, _P, _P, _P, _P]),
     ("mfvm_ns_dt", ctypes.c_int, [_G, _NP, _P, ctypes.c_double, _P, _P, _P]),
+    ("mfvm_ns_dt_noalloc", ctypes.c_int, [_G, _NP, _P, ctypes.c_double, _P, _P]),
     ("mfvm_ns_stage", ctypes.c_int, [_G, _NP, ctypes.POINTER(Stage), _P]),
     ("mfvm_ns_step", ctypes.c_int, [_G, _NP, ctypes.POINTER(NSWork), ctypes.c_int32,
                                     ctypes.c_double, _P]),
