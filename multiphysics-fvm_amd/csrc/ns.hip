@@ -65,26 +65,18 @@ __global__ __launch_bounds__(256, 2) void ns_stage_comp_kernel(StageArgs a) {
   ns_stage_body<LIM, RS, UNIF, float, true, false>(a, grid_tile(a), ptr_src<float>(a));
 }
 
-// Split stage (a.xy != NULL): x1/x2 faces, then x3 faces, sources and update.
-// FIXED: the 64 x 4 tile as compile-time constants and buffer-resource reads
-// (arrays below 4 GiB).  The LDS offsets fold into instruction immediates,
-// and far fewer scalar registers spill into vector lanes (float: 87 instead
-// of 110 VGPRs, five waves per SIMD instead of four).  Other tile shapes and
-// larger arrays take FIXED = false.
-//
-// SPEC (this kernel and the x3 kernels below): see spec_args.
+// Split stage (a.xy != NULL): x1/x2 faces (ns_xyrow_kernel), then x3 faces,
+// sources and update (ns_z_kernel / ns_z_comp_kernel).  SPEC: see spec_args.
 
-template <int LIM, int RS, bool UNIF, class R, bool FIXED, int SPEC = 0>
-__global__ __launch_bounds__(256, FIXED && sizeof(R) == 8 ? 3 : 1) void ns_xy_kernel(StageArgs a) {
+// Row-march form of the x1/x2 pass (ns_xyrow_body): XR_WAVES independent
+// waves per workgroup, one (strip, row chunk, plane) task each; BUF:
+// buffer-resource reads (arrays below 4 GiB).
+template <int LIM, int RS, bool UNIF, class R, bool BUF, int SPEC>
+__global__ __launch_bounds__(XR_W * XR_WAVES) void ns_xyrow_kernel(StageArgs a) {
   spec_args(a, SPEC);
-  TileCtx t = grid_tile(a);
-  if constexpr (FIXED) {
-    t.bx = 64;
-    t.by = 4;
-    ns_xy_body<LIM, RS, UNIF, R>(a, t, buf_src<R>(a));
-  } else {
-    ns_xy_body<LIM, RS, UNIF, R>(a, t, ptr_src<R>(a));
-  }
+  const long long task = (long long)blockIdx.x * XR_WAVES + threadIdx.x / XR_W;
+  if constexpr (BUF) ns_xyrow_body<LIM, RS, UNIF, R>(a, buf_src<R>(a), task);
+  else ns_xyrow_body<LIM, RS, UNIF, R>(a, ptr_src<R>(a), task);
 }
 
 // (Buffer-resource reads measure the same here as pointer reads.)
@@ -171,9 +163,37 @@ hipError_t launch_closure(const mfvm_grid* g, const mfvm_ns_params* p, const voi
   return hipGetLastError();
 }
 
-template <int LIM, int RS, bool UNIF, class R, bool FIXED, int SPEC>
-static void xy_launch(dim3 g, dim3 b, size_t lds, hipStream_t s, const StageArgs& a) {
-  hipLaunchKernelGGL((ns_xy_kernel<LIM, RS, UNIF, R, FIXED, SPEC>), g, b, lds, s, a);
+// Rows per chunk of the row-march x1/x2 pass: 16 (FP64 and FP32 both 1-6%
+// faster than 24 or 32 at 256^3); MFVM_XYROW_ROWS overrides (2 to XR_ROWS).
+static int xyrow_rows() {
+  static const int r = [] {
+    const char* e = getenv("MFVM_XYROW_ROWS");
+    const int v = e ? atoi(e) : 16;
+    return v < 2 ? 2 : (v > XR_ROWS ? XR_ROWS : v);
+  }();
+  return r;
+}
+
+template <int LIM, int RS, bool UNIF, class R, int SPEC>
+static void xyrow_launch(const StageArgs& a0, bool buf, hipStream_t s) {
+  StageArgs a = a0;
+  a.kz = xyrow_rows();
+  const long long ns = (a.lay.n1 + XR_W - 1) / XR_W;
+  const long long nc = (a.lay.n2 + a.kz - 1) / a.kz;
+  const int m1 = a.xm1 > 0 ? a.xm1 : a.lay.n3 + 1;
+  const long long ntask = ns * nc * (m1 - a.xm0);
+  const dim3 g((unsigned)((ntask + XR_WAVES - 1) / XR_WAVES)), b(XR_W * XR_WAVES);
+  const size_t lds = (size_t)XR_WAVES * XR_LDS_ELEMS * sizeof(R);
+  if (buf) hipLaunchKernelGGL((ns_xyrow_kernel<LIM, RS, UNIF, R, true, SPEC>), g, b, lds, s, a);
+  else hipLaunchKernelGGL((ns_xyrow_kernel<LIM, RS, UNIF, R, false, SPEC>), g, b, lds, s, a);
+}
+
+template <int LIM, int RS, class R>
+static void xyrow_any(const StageArgs& a, int spec, bool unif, bool buf, hipStream_t s) {
+  if (spec == 2) xyrow_launch<LIM, RS, true, R, 2>(a, buf, s);
+  else if (spec == 1) xyrow_launch<LIM, RS, true, R, 1>(a, buf, s);
+  else if (unif) xyrow_launch<LIM, RS, true, R, 0>(a, buf, s);
+  else xyrow_launch<LIM, RS, false, R, 0>(a, buf, s);
 }
 
 // The x3 pass: float64, or the compensated float32 form (final stage or not).
@@ -184,12 +204,6 @@ static void z_launch(bool comp, bool fin, dim3 g, dim3 b, size_t red, hipStream_
   else hipLaunchKernelGGL((ns_z_kernel<LIM, RS, UNIF, SPEC>), g, b, red, s, a);
 }
 
-// Planes per workgroup of the split x1/x2 pass, measured at 256^3.  float64:
-// 8 (8 and 4 time the same, 2 is 3% slower, 16 and 32 slower too); float32:
-// 2 (3% faster than 8, the same kernel being VALU-bound at five waves per
-// SIMD, where a shorter tail pays).
-constexpr int XY_KZ_F64 = 8, XY_KZ_F32 = 2;
-
 template <int LIM, int RS, class R>
 static hipError_t launch_stage_t(const StageArgs& a, bool unif, dim3 grd, dim3 blk, size_t lds, hipStream_t s) {
   const bool comp = a.comp != MFVM_COMP_NONE;
@@ -197,28 +211,16 @@ static hipError_t launch_stage_t(const StageArgs& a, bool unif, dim3 grd, dim3 b
     // split stage: x1/x2 faces (timed as the stage), then x3 faces, sources,
     // update and CFL (timed as "other")
     const size_t red = 2 * sizeof(double) * ((blk.x * blk.y + 63) / 64);
-    // The x1/x2 pass keeps nothing from one plane to the next, so its
-    // workgroups take short chunks of XY_KZ_* planes (more workgroups, a
-    // shorter tail); the x3 march keeps the longer chunk, which amortises
-    // its start-up planes.
     StageArgs ax = a;
-    const int xkz = sizeof(R) == sizeof(float) ? XY_KZ_F32 : XY_KZ_F64;
-    ax.kz = a.kz < xkz ? a.kz : xkz;
     ax.partials = nullptr;
-    const dim3 gx(grd.x, grd.y, (a.lay.n3 + ax.kz - 1) / ax.kz);
-    const bool fixed = blk.x == 64 && blk.y == 4 && buf_src_ok(a.lay, sizeof(R));
     const bool p2 = pow2_geo(a.geo);
     const int spec = unif && p2 ? (a.p.visc ? 2 : 1) : 0;
+    const bool buf = buf_src_ok(a.lay, sizeof(R));
+    const bool fin = a.comp >= MFVM_COMP_FINAL_SET;
     const int tx = t_begin(MFVM_TK_STAGE, s);
-    if (unif && fixed && spec == 2) xy_launch<LIM, RS, true, R, true, 2>(gx, blk, lds, s, ax);
-    else if (unif && fixed && spec == 1) xy_launch<LIM, RS, true, R, true, 1>(gx, blk, lds, s, ax);
-    else if (unif && fixed) xy_launch<LIM, RS, true, R, true, 0>(gx, blk, lds, s, ax);
-    else if (unif) xy_launch<LIM, RS, true, R, false, 0>(gx, blk, lds, s, ax);
-    else if (fixed) xy_launch<LIM, RS, false, R, true, 0>(gx, blk, lds, s, ax);
-    else xy_launch<LIM, RS, false, R, false, 0>(gx, blk, lds, s, ax);
+    xyrow_any<LIM, RS, R>(ax, spec, unif, buf, s);
     t_end(tx, s);
     const int tz = t_begin(MFVM_TK_OTHER, s);
-    const bool fin = a.comp >= MFVM_COMP_FINAL_SET;
     if (spec == 2) z_launch<LIM, RS, true, 2>(comp, fin, grd, blk, red, s, a);
     else if (spec == 1) z_launch<LIM, RS, true, 1>(comp, fin, grd, blk, red, s, a);
     else if (unif) z_launch<LIM, RS, true, 0>(comp, fin, grd, blk, red, s, a);

@@ -73,6 +73,8 @@ struct StageArgs {
   void* dinc;
   void* snext;
   int comp;
+  // The row-march x1/x2 pass covers planes [xm0, xm1) (xm1 = 0: to the last).
+  int xm0 = 1, xm1 = 0;
 };
 
 // ---------------------------------------------------------------------------
@@ -803,12 +805,18 @@ MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl, const S& src)
   bool src_any_prev = false;
   double rate_max = 0.0, nu_max = 0.0;
 
+  // The x3-only pass on a Cartesian grid reads T11, T22, T12 nowhere (the x3
+  // states take row 3 of T, the gravity source only U): it skips them.
+  const bool cart = UNIF || !a.geo.curv;
   auto load12 = [&](int k, R* w) {
     const long long o = a.lay.at(i, j, k);
 #pragma unroll
     for (int f = 0; f < 5; ++f) w[f] = src.u(o, f);
 #pragma unroll
-    for (int f = 0; f < NS_NC; ++f) w[5 + f] = (f == 0 || visc) ? src.c(o, f) : R(0);
+    for (int f = 0; f < NS_NC; ++f) {
+      const bool row12 = f == C_T11 || f == C_T22 || f == C_T12;
+      w[5 + f] = (f == 0 || (visc && !(ZONLY && cart && row12))) ? src.c(o, f) : R(0);
+    }
   };
   // ZS of a 12-vector: U, P, T13, T23, T33
   auto zs = [&](const R* w, R* z) {
@@ -1057,6 +1065,264 @@ MFVM_DEV void ns_xy_body(const StageArgs& a, const TileCtx& tl, const S& src) {
       for (int q = 0; q < 5; ++q) static_cast<R*>(a.xy)[oi + q * a.il.fs] = accm[q];
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// Row-march x1/x2 pass
+// ---------------------------------------------------------------------------
+// One wave advances a strip of 64 x1 cells (one per lane) through a chunk of
+// up to XR_ROWS x2 rows of one plane, marching along x2: the x2 stencil
+// window, the upper x2 state of the previous row and the previous x2 face
+// flux stay in registers, so each cell's x2 states and each x2 face flux are
+// formed once (two extra x2 states per chunk).  Along x1 each lane forms its
+// own cell's states; the lower state goes to the left neighbour and the face
+// flux to the right neighbour through this wave's LDS, with no workgroup
+// barrier (a wave's LDS operations complete in issue order).  The two faces
+// at the strip's ends are prepared for every row of the chunk at once: the
+// flux through face i0-1/2 and the lower state of cell i0+64.
+constexpr int XR_W = 64;      // x1 cells per strip (one wave)
+constexpr int XR_ROWS = 32;   // most x2 rows per chunk
+constexpr int XR_WAVES = 4;   // independent waves per workgroup
+
+// LDS of one wave (elements of R): the x1 exchange area X[NQ][64] -- each
+// row's x1 fields, then lower x1 states, then face fluxes (every lane's next
+// write depends on its reads of the previous contents) -- and per row of the
+// chunk the strip-end lower states EX[XR_ROWS][NQ], fluxes EF[XR_ROWS][5]
+// and x1 fields EQ[XR_ROWS][2][NQ] of cells i0-1 and i0+64.
+constexpr int XR_LDS_ELEMS = NQ * XR_W + XR_ROWS * NQ + XR_ROWS * 5 + XR_ROWS * 2 * NQ;
+
+// This wave's LDS writes are visible to its other lanes' later reads.
+MFVM_DEV void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The 11 stored quantities of cell o read by the x1/x2 pass: U (5), P, then
+// T11, T12, T13 (row 1) and T22, T23 (row 2 without T21 = T12).
+template <class R, class S>
+MFVM_DEV void load_xy11(const S& src, long long o, bool visc, R w[11]) {
+#pragma unroll
+  for (int f = 0; f < 5; ++f) w[f] = src.u(o, f);
+  w[5] = src.c(o, C_P);
+  w[6] = visc ? src.c(o, C_T11) : R(0);
+  w[7] = visc ? src.c(o, C_T12) : R(0);
+  w[8] = visc ? src.c(o, C_T13) : R(0);
+  w[9] = visc ? src.c(o, C_T22) : R(0);
+  w[10] = visc ? src.c(o, C_T23) : R(0);
+}
+// Reconstruction fields along x1 / x2 from the 11 quantities.
+template <class R>
+MFVM_DEV void q_x1(const R w[11], R q[NQ]) {
+#pragma unroll
+  for (int f = 0; f < 9; ++f) q[f] = w[f];
+}
+template <class R>
+MFVM_DEV void q_x2(const R w[11], R q[NQ]) {
+#pragma unroll
+  for (int f = 0; f < 6; ++f) q[f] = w[f];
+  q[6] = w[7];
+  q[7] = w[9];
+  q[8] = w[10];
+}
+
+// x2 states of ghosted cell (i, j, m) from its x2 window (wm, w, wp: the
+// reconstruction fields of rows j-1, j, j+1); ghost rows are unsloped.
+template <int LIM, bool UNIF, class R>
+MFVM_DEV void x2_states(const StageArgs& a, int i, int j, int m, const R wm[NQ], const R w[NQ], const R wp[NQ],
+                        R lo[NQ], R hi[NQ]) {
+  if (LIM == MFVM_LIM_NONE || j == 0 || j == a.lay.n2 + 1) {
+#pragma unroll
+    for (int f = 0; f < NQ; ++f) lo[f] = hi[f] = w[f];
+    return;
+  }
+  slope_states<1, LIM, UNIF, R>(a.geo, i, j, m, a.p.visc ? 9 : 6, wm, w, wp, lo, hi);
+}
+
+// One wave's strip: x1 cells i0 .. i0+63, rows jb .. je-1 of plane m; the
+// in-plane divergence of every own cell goes to a.xy.  lds: this wave's
+// XR_LDS_ELEMS elements.  Every lane runs the same instructions: lanes past
+// the strip's last interior cell read the x1 ghost column instead (their
+// results are never stored), so no state carried from row to row depends on
+// a branch.
+template <int LIM, int RS, bool UNIF, class R, class S>
+MFVM_DEV void ns_xyrow_strip(const StageArgs& a, const S& src, R* __restrict__ lds, int i0, int jb, int je, int m) {
+  R* XR = lds;                    // [NQ][64]
+  R* XF = lds;                    // [5][64], over XR
+  R* EX = XR + NQ * XR_W;         // [XR_ROWS][NQ]
+  R* EF = EX + XR_ROWS * NQ;      // [XR_ROWS][5]
+  R* EQ = EF + XR_ROWS * 5;       // [XR_ROWS][2][NQ]
+  R* XQ = lds;                    // [NQ][64], x1 fields, over XR
+  const int n1 = a.lay.n1, n2 = a.lay.n2;
+  const int l = threadIdx.x & (XR_W - 1);
+  const int i = i0 + l;
+  const bool own = i <= n1;             // interior cell of this lane
+  const int ic = min(i, n1 + 1);        // column read by this lane
+  const int ig = min(i, n1);            // column of its metric factors (a valid cell)
+  const bool visc = a.p.visc != 0;
+  const int nr = je - jb;
+
+  // ---- strip ends, every row of the chunk ---------------------------------------
+  {
+    // pass 1: lanes [0, 32) the upper x1 state of cell (i0-1, jb+l), lanes
+    // [32, 64) the lower x1 state of cell (i0+64, jb+l-32) (rows and the
+    // column clamped into the array; only the chunk's rows are kept)
+    const bool left = l < 32;
+    const int r = left ? l : l - 32;
+    const int cj = min(jb + r, je - 1);
+    const int ci = left ? i0 - 1 : min(i0 + XR_W, n1 + 1);
+    R lo[NQ], hi[NQ], qe[NQ];
+    cell_states<0, LIM, UNIF, R>(a, src, ci, cj, m, lo, hi);
+    load_q<0, R>(src, a.lay.at(ci, cj, m), visc, qe);
+    if (r < nr) {
+#pragma unroll
+      for (int f = 0; f < NQ; ++f) EQ[(2 * r + (left ? 0 : 1)) * NQ + f] = qe[f];
+    }
+    if (!left && r < nr) {
+#pragma unroll
+      for (int f = 0; f < NQ; ++f) EX[r * NQ + f] = lo[f];
+    }
+    // pass 2: the lower x1 state of cell (i0, jb+l) and the flux through
+    // face i0-1/2 (kept from lanes [0, nr))
+    R lo0[NQ], hi0[NQ], F[5];
+    cell_states<0, LIM, UNIF, R>(a, src, i0, cj, m, lo0, hi0);
+    face_flux<0, RS, R>(a.p, hi, lo0, g_area<UNIF, R>(a.geo, 0, i0 - 1, cj - 1, m - 1), F);
+    if (left && r < nr) {
+#pragma unroll
+      for (int q = 0; q < 5; ++q) EF[r * 5 + q] = F[q];
+    }
+  }
+  wave_lds_sync();
+
+  // ---- x2 march start: states of row jb-1 -----------------------------------------
+  // rows j-1 (x2 fields), j, j+1 and j+2 (loaded two rows ahead of use)
+  R wm[NQ], cw[11], cn[11], cnn[11], hi2p[NQ], Gp[5], axp[5];
+  {
+    R c11[11];
+    load_xy11<R>(src, a.lay.at(ic, jb - 1, m), visc, c11);
+    q_x2<R>(c11, wm);
+    load_xy11<R>(src, a.lay.at(ic, jb, m), visc, cw);
+    load_xy11<R>(src, a.lay.at(ic, min(jb + 1, n2 + 1), m), visc, cn);
+    load_xy11<R>(src, a.lay.at(ic, min(jb + 2, n2 + 1), m), visc, cnn);
+    if (jb - 1 == 0 || LIM == MFVM_LIM_NONE) {
+#pragma unroll
+      for (int f = 0; f < NQ; ++f) hi2p[f] = wm[f];
+    } else {
+      R wmm[NQ], w0[NQ], lo[NQ];
+      load_xy11<R>(src, a.lay.at(ic, jb - 2, m), visc, c11);
+      q_x2<R>(c11, wmm);
+      q_x2<R>(cw, w0);
+      x2_states<LIM, UNIF, R>(a, ig, jb - 1, m, wmm, wm, w0, lo, hi2p);
+    }
+  }
+  const long long ifs = a.il.fs;
+  R* xy = static_cast<R*>(a.xy);
+  // One row of the march; true once row je (the row after the chunk) has
+  // closed the divergence of row je-1.  The loop runs it twice per trip, so
+  // that the carried rows and states swap registers instead of being copied.
+  auto row = [&](int j) -> bool {
+    // ---- x2: states of row j, face j-1/2, the divergence of row j-1 ------------------
+    {
+      R w[NQ], wp[NQ], lo2[NQ], hi2[NQ], G[5];
+      q_x2<R>(cw, w);
+      q_x2<R>(cn, wp);  // row j+1 (unused when row j is the ghost row)
+      x2_states<LIM, UNIF, R>(a, ig, j, m, wm, w, wp, lo2, hi2);
+      face_flux<1, RS, R>(a.p, hi2p, lo2, g_area<UNIF, R>(a.geo, 1, ig - 1, j - 1, m - 1), G);
+      if (own && j > jb) {
+        const long long oi = a.il.at(i, j - 1, m);
+#pragma unroll
+        for (int q = 0; q < 5; ++q) xy[oi + q * ifs] = (axp[q] + G[q]) - Gp[q];
+      }
+#pragma unroll
+      for (int q = 0; q < 5; ++q) Gp[q] = G[q];
+#pragma unroll
+      for (int f = 0; f < NQ; ++f) {
+        hi2p[f] = hi2[f];
+        wm[f] = w[f];
+      }
+    }
+    if (j == je) return true;
+    // ---- x1 faces of row j: states, exchange through LDS, flux differences ---------
+    {
+      wave_lds_sync();  // the previous row's reads of XF are done
+      R q[NQ], lo1[NQ], hi1[NQ];
+      q_x1<R>(cw, q);
+      if (LIM == MFVM_LIM_NONE) {
+#pragma unroll
+        for (int f = 0; f < NQ; ++f) lo1[f] = hi1[f] = q[f];
+      } else {
+        // x1 neighbours from the adjacent lanes (the strip ends from EQ); the
+        // ghost column's neighbours are itself: zero differences, zero slopes
+        // over a valid cell length, raw states
+#pragma unroll
+        for (int f = 0; f < NQ; ++f) XQ[f * XR_W + l] = q[f];
+        wave_lds_sync();
+        R qm[NQ], qp[NQ];
+        const bool in = i <= n1;
+        const R* pm = !in ? XQ + l : (l == 0 ? EQ + 2 * (j - jb) * NQ : XQ + l - 1);
+        const R* pp = !in ? XQ + l : (l == XR_W - 1 ? EQ + (2 * (j - jb) + 1) * NQ : XQ + l + 1);
+        const int sm_ = (in && l == 0) ? 1 : XR_W;
+        const int sp_ = (in && l == XR_W - 1) ? 1 : XR_W;
+#pragma unroll
+        for (int f = 0; f < NQ; ++f) {
+          qm[f] = pm[f * sm_];
+          qp[f] = pp[f * sp_];
+        }
+        slope_states<0, LIM, UNIF, R>(a.geo, ig, j, m, visc ? 9 : 6, qm, q, qp, lo1, hi1);
+      }
+#pragma unroll
+      for (int f = 0; f < NQ; ++f) XR[f * XR_W + l] = lo1[f];
+      wave_lds_sync();
+      R rr[NQ], F[5];
+      const R* src_r = l == XR_W - 1 ? EX + (j - jb) * NQ : XR + l + 1;
+      const int st = l == XR_W - 1 ? 1 : XR_W;
+#pragma unroll
+      for (int f = 0; f < NQ; ++f) rr[f] = src_r[f * st];
+      face_flux<0, RS, R>(a.p, hi1, rr, g_area<UNIF, R>(a.geo, 0, ig, j - 1, m - 1), F);
+#pragma unroll
+      for (int q2 = 0; q2 < 5; ++q2) XF[q2 * XR_W + l] = F[q2];
+      wave_lds_sync();
+      const R* src_l = l == 0 ? EF + (j - jb) * 5 : XF + l - 1;
+      const int sl = l == 0 ? 1 : XR_W;
+#pragma unroll
+      for (int q2 = 0; q2 < 5; ++q2) axp[q2] = F[q2] - src_l[q2 * sl];
+    }
+    // ---- next row: j+1 becomes the centre, j+3 loads ---------------------------------
+#pragma unroll
+    for (int f = 0; f < 11; ++f) {
+      cw[f] = cn[f];
+      cn[f] = cnn[f];
+    }
+    load_xy11<R>(src, a.lay.at(ic, min(j + 3, n2 + 1), m), visc, cnn);
+    return false;
+  };
+  for (int j = jb;; j += 2) {
+    if (row(j) || row(j + 1)) break;
+  }
+}
+
+// x1/x2 pass of a split stage in row-march form: workgroups of XR_WAVES
+// independent waves, one (strip, row chunk, plane) task per wave; rows per
+// chunk a.kz (<= XR_ROWS).
+template <int LIM, int RS, bool UNIF, class R, class S>
+MFVM_DEV void ns_xyrow_body(const StageArgs& a, const S& src, long long task) {
+  extern __shared__ double smem_raw[];
+  const int wave = threadIdx.x / XR_W;
+  R* lds = reinterpret_cast<R*>(smem_raw) + wave * XR_LDS_ELEMS;
+  const int ns = (a.lay.n1 + XR_W - 1) / XR_W;
+  const int rows = a.kz;
+  const int nc = (a.lay.n2 + rows - 1) / rows;
+  const int m1 = a.xm1 > 0 ? a.xm1 : a.lay.n3 + 1;
+  const long long ntask = (long long)ns * nc * (m1 - a.xm0);
+  if (task >= ntask) return;
+  const int s = (int)(task % ns);
+  const long long rest = task / ns;
+  const int c = (int)(rest % nc);
+  const int m = a.xm0 + (int)(rest / nc);
+  const int jb = 1 + c * rows;
+  const int je = min(jb + rows, a.lay.n2 + 1);
+  ns_xyrow_strip<LIM, RS, UNIF, R>(a, src, lds, 1 + s * XR_W, jb, je, m);
 }
 
 // CFL maxima of one tile (navier_stokes.py:512-583): rate_c and nu over the
