@@ -793,7 +793,15 @@ MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl, const S& src)
   const int kb = 1 + tl.z * a.kz;
   const int ke = min(kb + a.kz, n3 + 1);
   const bool own = (i <= n1) && (j <= n2);
+  // Every lane marches: a lane past the grid's edge reads and computes the
+  // nearest interior column instead (its results are never stored), so that
+  // no state carried from plane to plane depends on a branch.
+  const int ic = min(i, n1), jc = min(j, n2);
   const bool visc = a.p.visc != 0;
+  // LATE (x3-only pass on a uniform Cartesian grid): the in-plane divergence
+  // and the source of cell m-1 are read and formed when it is finalized
+  // (the source needs only U there), instead of being carried a plane.
+  constexpr bool LATE = ZONLY && UNIF;
 
   double dt = a.dt_dev ? *a.dt_dev : a.dt;
 
@@ -809,7 +817,7 @@ MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl, const S& src)
   // states take row 3 of T, the gravity source only U): it skips them.
   const bool cart = UNIF || !a.geo.curv;
   auto load12 = [&](int k, R* w) {
-    const long long o = a.lay.at(i, j, k);
+    const long long o = a.lay.at(ic, jc, k);
 #pragma unroll
     for (int f = 0; f < 5; ++f) w[f] = src.u(o, f);
 #pragma unroll
@@ -827,7 +835,7 @@ MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl, const S& src)
     z[8] = w[5 + C_T33];
   };
 
-  if (own) {
+  {
     R w0[12];
     load12(kb - 1, w0);
     zs(w0, zp);
@@ -840,7 +848,7 @@ MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl, const S& src)
       load12(kb - 2, wm);
       zs(wm, zm);
       zs(cu, zc);
-      slope_states<2, LIM, UNIF, R>(a.geo, i, j, kb - 1, visc ? 9 : 6, zm, zp, zc, lo, lz_prev);
+      slope_states<2, LIM, UNIF, R>(a.geo, ic, jc, kb - 1, visc ? 9 : 6, zm, zp, zc, lo, lz_prev);
     }
     // The x3-only pass has no barriers between planes: it loads each plane
     // one iteration ahead, so that the loads overlap the previous plane's work.
@@ -852,7 +860,7 @@ MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl, const S& src)
     R fsz[5];
     R lz_cur[NQ];
     R nn[12];  // ZONLY: plane m+2, loaded now for the next iteration
-    if (own) {
+    {
       R zc[NQ], lo[NQ];
       zs(cu, zc);
       if constexpr (ZONLY) {
@@ -866,9 +874,9 @@ MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl, const S& src)
       } else {
         R zn[NQ];
         zs(nx, zn);
-        slope_states<2, LIM, UNIF, R>(a.geo, i, j, m, visc ? 9 : 6, zp, zc, zn, lo, lz_cur);
+        slope_states<2, LIM, UNIF, R>(a.geo, ic, jc, m, visc ? 9 : 6, zp, zc, zn, lo, lz_cur);
       }
-      const R s3 = g_area<UNIF, R>(a.geo, 2, i - 1, j - 1, m - 1);
+      const R s3 = g_area<UNIF, R>(a.geo, 2, ic - 1, jc - 1, m - 1);
       face_flux<2, RS, R>(a.p, lz_prev, lo, s3, fsz);
     }
 
@@ -879,34 +887,39 @@ MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl, const S& src)
     // phase 1 and to the last threads in phase 2, so that no wave takes two.
     R accm[5], srcm[5];
     bool src_any = false;
-    if (m < ke) {
+    if (!LATE && m < ke) {
       if constexpr (ZONLY) {
-        if (own) {
-          const long long oi = a.il.at(i, j, m);
+        const long long oi = a.il.at(ic, jc, m);
 #pragma unroll
-          for (int q = 0; q < 5; ++q) accm[q] = static_cast<const R*>(a.xy)[oi + q * a.il.fs];
-        }
+        for (int q = 0; q < 5; ++q) accm[q] = static_cast<const R*>(a.xy)[oi + q * a.il.fs];
       } else {
         inplane_div<LIM, RS, UNIF, R>(a, src, sm, BX, BY, tx, ty, tid, i0, j0, i, j, own, m, cu, accm);
       }
-      if (own) cell_source<UNIF, R>(a.geo, a.p, i, j, m, cu, cu + 5, srcm, src_any);
+      cell_source<UNIF, R>(a.geo, a.p, ic, jc, m, cu, cu + 5, srcm, src_any);
     }
 
-    // ---- finalize cell m-1 -----------------------------------------------------
-    if (own && m > kb) {
+    // ---- finalize cell m-1 (stores from the lanes of interior columns) --------
+    if (m > kb) {
       const int k = m - 1;
       R kr[5];
       R vol;
       const bool vp2 = a.geo.vpow2 != 0;
-      if (!vp2) vol = g_vol<UNIF, R>(a.geo, i, j, k);
+      if (!vp2) vol = g_vol<UNIF, R>(a.geo, ic, jc, k);
+      if constexpr (LATE) {
+        const long long oi = a.il.at(ic, jc, k);
+#pragma unroll
+        for (int q = 0; q < 5; ++q) acc_prev[q] = static_cast<const R*>(a.xy)[oi + q * a.il.fs];
+        // (uniform Cartesian: the source reads U only, never the closure)
+        cell_source<UNIF, R>(a.geo, a.p, ic, jc, k, zp, zp + 5, src_prev, src_any_prev);
+      }
 #pragma unroll
       for (int q = 0; q < 5; ++q) {
         const R x = -((acc_prev[q] + fsz[q]) - fsz_prev[q]);
         kr[q] = vp2 ? x * R(a.geo.vinv) : qdiv(x, vol);
         if (src_any_prev) kr[q] = kr[q] + src_prev[q];
       }
-      const long long oc = a.lay.at(i, j, k);
-      const long long oi = a.il.at(i, j, k);
+      const long long oc = a.lay.at(ic, jc, k);
+      const long long oi = a.il.at(ic, jc, k);
       R* out = static_cast<R*>(a.out);
       R* ka = static_cast<R*>(a.kacc);
       const R* U0 = static_cast<const R*>(a.U0);
@@ -935,15 +948,17 @@ MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl, const S& src)
             default: break;
           }
           nu_[q] = o;
-          if (a.update != MFVM_UPD_NONE) out[oc + q * fs] = o;
-          if (a.acc == MFVM_ACC_SET) ka[oi + q * a.il.fs] = kr[q];
-          else if (a.acc == MFVM_ACC_ADD) ka[oi + q * a.il.fs] = kav[q] + R(a.acc_coef) * kr[q];
+          if (own) {
+            if (a.update != MFVM_UPD_NONE) out[oc + q * fs] = o;
+            if (a.acc == MFVM_ACC_SET) ka[oi + q * a.il.fs] = kr[q];
+            else if (a.acc == MFVM_ACC_ADD) ka[oi + q * a.il.fs] = kav[q] + R(a.acc_coef) * kr[q];
+          }
         }
         if (a.partials) {
           R rate, nu;
-          cell_cfl<UNIF, R>(a.geo, a.p, i, j, k, nu_, rate, nu);
-          rate_max = nmax(rate_max, double(rate));
-          nu_max = nmax(nu_max, double(nu));
+          cell_cfl<UNIF, R>(a.geo, a.p, ic, jc, k, nu_, rate, nu);
+          rate_max = own ? nmax(rate_max, double(rate)) : rate_max;
+          nu_max = own ? nmax(nu_max, double(nu)) : nu_max;
         }
       } else {
         // Compensated update: U0 = hi + lo (two FP32 words).  Stage states
@@ -976,8 +991,10 @@ MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl, const S& src)
             R d;
             if (a.comp == MFVM_COMP_SET) d = R(a.c0 * dt) * kq;
             else d = R(a.c0) * (dvv[q] + R(dt) * kq);
-            dv[ii] = d;
-            sn[og] = h + d;
+            if (own) {
+              dv[ii] = d;
+              sn[og] = h + d;
+            }
           } else {
             double inc;
             if (a.comp == MFVM_COMP_FINAL_SET) inc = (a.c0 * dt) * double(kq);
@@ -985,29 +1002,33 @@ MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl, const S& src)
             else inc = (dt / 6.0) * (double(kav[q]) + double(kq));
             const double v = (double(h) + double(l)) + inc;
             const R nh = R(v);
-            out[og] = nh;
-            lo[og] = R(v - double(nh));
+            if (own) {
+              out[og] = nh;
+              lo[og] = R(v - double(nh));
+            }
             nw[q] = v;
           }
-          if (a.acc == MFVM_ACC_SET) ka[ii] = kq;
-          else if (a.acc == MFVM_ACC_ADD) ka[ii] = kav[q] + R(a.acc_coef) * kq;
+          if (own) {
+            if (a.acc == MFVM_ACC_SET) ka[ii] = kq;
+            else if (a.acc == MFVM_ACC_ADD) ka[ii] = kav[q] + R(a.acc_coef) * kq;
+          }
         }
         if (fin && a.partials) {
           double rate, nu;
-          cell_cfl<UNIF, double>(a.geo, a.p, i, j, k, nw, rate, nu);
-          rate_max = nmax(rate_max, rate);
-          nu_max = nmax(nu_max, nu);
+          cell_cfl<UNIF, double>(a.geo, a.p, ic, jc, k, nw, rate, nu);
+          rate_max = own ? nmax(rate_max, rate) : rate_max;
+          nu_max = own ? nmax(nu_max, nu) : nu_max;
         }
       }
     }
 
     // ---- rotate the window -------------------------------------------------------
-    if (own) {
+    {
 #pragma unroll
       for (int q = 0; q < 5; ++q) fsz_prev[q] = fsz[q];
 #pragma unroll
       for (int f = 0; f < NQ; ++f) lz_prev[f] = lz_cur[f];
-      if (m < ke) {
+      if (!LATE && m < ke) {
 #pragma unroll
         for (int q = 0; q < 5; ++q) {
           acc_prev[q] = accm[q];
