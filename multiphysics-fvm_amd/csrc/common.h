@@ -147,6 +147,16 @@ MFVM_DEV TileId tile_id(bool remap) {
   return t;
 }
 
+// The same renumbering for a one-dimensional grid: XCD x runs a contiguous
+// range of block numbers.
+MFVM_DEV unsigned xcd_block_1d(bool remap) {
+  const unsigned w = blockIdx.x;
+  if (!remap) return w;
+  const unsigned nb = gridDim.x;
+  const unsigned q = nb >> 3, r = nb & 7u, x = w & 7u;
+  return x * q + min(x, r) + (w >> 3);
+}
+
 // XCD-aware tile order (tile_id) on the host side; MFVM_XCD_REMAP=0 selects
 // the plain order.
 inline int xcd_remap_enabled() {

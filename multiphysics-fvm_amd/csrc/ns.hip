@@ -74,7 +74,9 @@ __global__ __launch_bounds__(256, 2) void ns_stage_comp_kernel(StageArgs a) {
 template <int LIM, int RS, bool UNIF, class R, bool BUF, int SPEC>
 __global__ __launch_bounds__(XR_W * XR_WAVES) void ns_xyrow_kernel(StageArgs a) {
   spec_args(a, SPEC);
-  const long long task = (long long)blockIdx.x * XR_WAVES + threadIdx.x / XR_W;
+  // consecutive workgroups (neighbouring row chunks of a plane, whose halo
+  // rows overlap) on one XCD, sharing its L2
+  const long long task = (long long)xcd_block_1d(a.remap != 0) * XR_WAVES + threadIdx.x / XR_W;
   if constexpr (BUF) ns_xyrow_body<LIM, RS, UNIF, R>(a, buf_src<R>(a), task);
   else ns_xyrow_body<LIM, RS, UNIF, R>(a, ptr_src<R>(a), task);
 }

@@ -370,12 +370,19 @@ template <int D, int LIM, bool UNIF, class R>
 MFVM_DEV void slope_states(const Geo& g, int i, int j, int k, int nf, const R qm[NQ],
                            const R q[NQ], const R qp[NQ], R lo[NQ], R hi[NQ]) {
   const R len = cell_len<D, UNIF, R>(g, i, j, k);
+  // float32 over a power-of-two length: every limiter is positively
+  // homogeneous of degree one and scaling by a power of two is exact, so
+  // limiter(db / L, df / L) * L is formed as limiter(db, df) (equal but for
+  // the underflow range of the limiter's products).  float64 keeps the
+  // reference's operations (slope_limiters.py, reconstructors.py:120-148).
+  const bool raw = sizeof(R) == sizeof(float) && g.lpow2[D];
 #pragma unroll
   for (int f = 0; f < NQ; ++f) {
     if (f < nf) {
-      const R b = div_len<R>(g, D, q[f] - qm[f], len);
-      const R fw = div_len<R>(g, D, qp[f] - q[f], len);
-      const R s = limit<LIM, R>(b, fw) * len;
+      const R db = q[f] - qm[f], df = qp[f] - q[f];
+      const R b = raw ? db : div_len<R>(g, D, db, len);
+      const R fw = raw ? df : div_len<R>(g, D, df, len);
+      const R s = raw ? limit<LIM, R>(b, fw) : limit<LIM, R>(b, fw) * len;
       hi[f] = q[f] + s / R(2);
       lo[f] = q[f] - s / R(2);
     } else {
