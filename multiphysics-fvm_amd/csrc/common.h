@@ -62,16 +62,20 @@ template <int LIM, class R> MFVM_DEV R limit(R b, R f) {
     const bool opp = ((b > R(0)) & (f < R(0))) | ((b < R(0)) & (f > R(0)));
     const R third = copysign(R(1.0 / 3.0) * least, c);
     return same ? copysign(least, b) : (opp ? third : zero);
+  } else if constexpr (sizeof(R) == sizeof(float)) {
+    // float32 van Leer: (2b)|f| * rcp(den) when b f > 0, else 0.  For equal
+    // signs this is the reference magnitude (2b)f/den with the sign of b, as
+    // the float64 branch below forms it, in fewer instructions; a NaN input
+    // gives 0 here.  The hardware reciprocal of a denominator below 2^-126 is
+    // inf; fields that start at exactly zero (Maxwell's B) pass through that
+    // range, so the denominator is raised to 2^-100: the harmonic mean is
+    // below half of it, so both it and the true value are then below 2^-101.
+    const R den = fmaxf(fabsf(b) + fabsf(f), 0x1p-100f);
+    const R harm = ((b + b) * fabsf(f)) * __builtin_amdgcn_rcpf(den);
+    return (b * f > R(0)) ? harm : R(0);
   } else {
     // slope_limiters.py:94-103 (harmonic mean on denom > 0, else 0)
-    R den = fabs(b) + fabs(f);
-    // float32: the hardware reciprocal of a denominator below 2^-126 is inf
-    // (and (2 b f) underflows to 0, so the product is NaN).  Fields that start
-    // at exactly zero (Maxwell's B) pass through that range.  The harmonic
-    // mean is below half the denominator, so with the denominator raised to
-    // 2^-100 both it and the true value are below 2^-101.  (A NaN slope still
-    // yields NaN: `same` is false and `zero` carries it.)
-    if constexpr (sizeof(R) == sizeof(float)) den = fmaxf(den, 0x1p-100f);
+    const R den = fabs(b) + fabs(f);
     const R harm = qdiv(((R(2) * b) * f), den);
     return same ? copysign(harm, b) : zero;
   }

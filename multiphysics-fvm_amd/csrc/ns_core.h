@@ -445,8 +445,15 @@ MFVM_DEV void face_flux(const NSPar& p, const R l[NQ], const R r[NQ], R area, R 
   R F[5];
   if constexpr (RS == MFVM_RS_HLL) {
     // Wave set {vn, vn-cs, vn+cs}: its min is vn-cs and its max vn+cs.
-    const R smin = nmin(vl[D] - csl, vr[D] - csr);
-    const R smax = nmax(vl[D] + csl, vr[D] + csr);
+    // (float32: the hardware min / max, equal but for NaN operands.)
+    R smin, smax;
+    if constexpr (sizeof(R) == sizeof(float)) {
+      smin = fminf(vl[D] - csl, vr[D] - csr);
+      smax = fmaxf(vl[D] + csl, vr[D] + csr);
+    } else {
+      smin = nmin(vl[D] - csl, vr[D] - csr);
+      smax = nmax(vl[D] + csl, vr[D] + csr);
+    }
     if (smin >= R(0)) {
 #pragma unroll
       for (int q = 0; q < 5; ++q) F[q] = fl[q];
