@@ -7,9 +7,10 @@ over the 256^3 grid: MUSCL + van Leer, HLL, centred viscous flux
 in HBM before the timed region; dt stays on the device.
 
 Prints one JSON line (metric Mcell-updates/s) with the roofline of the
-dominant kernel (the x1/x2 half of each stage, HIP events around each launch
-inside the timed region) and the CPU baseline (the NumPy oracle on a bounded sample
-of the same scheme, timed on this host).
+dominant kernel (the x1/x2 pass of each stage, ns_xyrow_kernel, timed with
+HIP events around each launch inside the timed region, against the
+compulsory 320/3 B per cell and stage) and the CPU baseline (the NumPy oracle
+on a bounded sample of the same scheme, timed on this host).
 """
 
 import argparse
@@ -32,18 +33,18 @@ HBM_PEAK_GBS = 8000.0
 # of it, 320/3 B per cell and stage.
 BYTES_PER_CELL_STEP = 320.0
 STAGES = 3
-# Bytes per cell that the split x1/x2 kernel itself moves (read U 5 x 8 B and
-# the closure P, T11..T23 7 x 8 B, write the in-plane divergence 5 x 8 B):
-# reported beside the roofline, not used for it.
-XY_OWN_BYTES_PER_CELL = 136.0
+# Bytes per cell that the x1/x2 kernel itself must move (read U 5 x 8 B and
+# the closure P, T11, T12, T13, T22, T23 6 x 8 B, write the in-plane
+# divergence 5 x 8 B): reported beside the roofline, not used for it.
+XY_OWN_BYTES_PER_CELL = 128.0
 # HBM bytes per launch of the dominant kernel at 256^3 (default tile), from the
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs summarised in the named
 # profile (KiB per dispatch, scaled by the 1 GiB copy calibration: read =
 # FETCH_SIZE * 1024 * 2.0, written = WRITE_SIZE * 1024 * 1.0).  The profiled
 # kernel build is named with it; None where no such profile exists.
 PROFILED_TRAFFIC = {
-    ("ns_xy_kernel", 256): ((824524 * 2.0 + 655360 * 1.0) * 1024,
-                            "profiles/ns_xy_kernel_rt256.md: prof90 FETCH_SIZE / WRITE_SIZE runs"),
+    ("ns_xyrow_kernel", 256): ((1150642 * 2.0 + 655360 * 1.0) * 1024,
+                               "profiles/ns_xyrow_kernel_rt256.md: prof_b FETCH_SIZE / WRITE_SIZE runs"),
 }
 
 
@@ -184,7 +185,7 @@ def main():
     names = ("stage_xy" if split else "stage", "closure", "bc", "stage_z" if split else "other")
     stage_ms = kms[_lib.TK_STAGE] / max(kcnt[_lib.TK_STAGE], 1)
     z_ms = kms[_lib.TK_OTHER] / max(kcnt[_lib.TK_OTHER], 1) if split else 0.0
-    kernel = "ns_xy_kernel" if split else "ns_stage_kernel"
+    kernel = "ns_xyrow_kernel" if split else "ns_stage_kernel"
     # roofline of the dominant kernel against the compulsory 320/3 B per cell
     # and stage (one launch advances every cell through one stage's share)
     stage_bytes = BYTES_PER_CELL_STEP / STAGES * cells
