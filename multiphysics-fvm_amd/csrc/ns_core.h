@@ -816,6 +816,7 @@ MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl, const S& src)
   // and the source of cell m-1 are read and formed when it is finalized
   // (the source needs only U there), instead of being carried a plane.
   constexpr bool LATE = ZONLY && UNIF;
+  constexpr bool EARLY = ZONLY && !COMP && LIM != MFVM_LIM_NONE;
 
   double dt = a.dt_dev ? *a.dt_dev : a.dt;
 
@@ -874,6 +875,34 @@ MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl, const S& src)
     R fsz[5];
     R lz_cur[NQ];
     R nn[12];  // ZONLY: plane m+2, loaded now for the next iteration
+    // EARLY: the reads of the cell finalized in this iteration (m-1) -- U0,
+    // kacc and, in the LATE form, the in-plane divergence -- are issued here,
+    // before the x3 face's work, so that their latency overlaps it (the
+    // float64 x3-only pass stays at 2 waves per SIMD).  The other forms issue
+    // them at the finalize (early reads would cost the compensated x3 pass a
+    // wave per SIMD).
+    R fxy[5], f0[5], f1[5];
+    auto xy_reads = [&](int k) {
+      const long long oi = a.il.at(ic, jc, k);
+#pragma unroll
+      for (int q = 0; q < 5; ++q) fxy[q] = static_cast<const R*>(a.xy)[oi + q * a.il.fs];
+    };
+    auto base_reads = [&](int k) {
+      const long long oc = a.lay.at(ic, jc, k), oi = a.il.at(ic, jc, k);
+      const bool need0 = a.update == MFVM_UPD_BASE || a.update == MFVM_UPD_SSP || a.update == MFVM_UPD_ACC6;
+      const bool needk = a.update == MFVM_UPD_ACC6 || a.acc == MFVM_ACC_ADD;
+#pragma unroll
+      for (int q = 0; q < 5; ++q) {
+        f0[q] = need0 ? static_cast<const R*>(a.U0)[oc + q * fs] : R(0);
+        f1[q] = needk ? static_cast<const R*>(a.kacc)[oi + q * a.il.fs] : R(0);
+      }
+    };
+    if constexpr (EARLY) {
+      if (m > kb) {
+        if constexpr (LATE) xy_reads(m - 1);
+        base_reads(m - 1);
+      }
+    }
     {
       R zc[NQ], lo[NQ];
       zs(cu, zc);
@@ -920,9 +949,14 @@ MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl, const S& src)
       const bool vp2 = a.geo.vpow2 != 0;
       if (!vp2) vol = g_vol<UNIF, R>(a.geo, ic, jc, k);
       if constexpr (LATE) {
-        const long long oi = a.il.at(ic, jc, k);
+        if constexpr (EARLY) {
 #pragma unroll
-        for (int q = 0; q < 5; ++q) acc_prev[q] = static_cast<const R*>(a.xy)[oi + q * a.il.fs];
+          for (int q = 0; q < 5; ++q) acc_prev[q] = fxy[q];
+        } else {
+          const long long oi = a.il.at(ic, jc, k);
+#pragma unroll
+          for (int q = 0; q < 5; ++q) acc_prev[q] = static_cast<const R*>(a.xy)[oi + q * a.il.fs];
+        }
         // (uniform Cartesian: the source reads U only, never the closure)
         cell_source<UNIF, R>(a.geo, a.p, ic, jc, k, zp, zp + 5, src_prev, src_any_prev);
       }
@@ -936,19 +970,27 @@ MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl, const S& src)
       const long long oi = a.il.at(ic, jc, k);
       R* out = static_cast<R*>(a.out);
       R* ka = static_cast<R*>(a.kacc);
-      const R* U0 = static_cast<const R*>(a.U0);
       // The cell's reads of U0, kacc, lo and dinc are all issued before its
       // first store (the arrays do not alias the outputs), so that they take
       // one memory round trip instead of one per field.
       if constexpr (!COMP) {
         R nu_[5], u0v[5], kav[5];
         const R dtr = R(dt);
-        const bool need0 = a.update == MFVM_UPD_BASE || a.update == MFVM_UPD_SSP || a.update == MFVM_UPD_ACC6;
-        const bool needk = a.update == MFVM_UPD_ACC6 || a.acc == MFVM_ACC_ADD;
+        if constexpr (EARLY) {
 #pragma unroll
-        for (int q = 0; q < 5; ++q) {
-          u0v[q] = need0 ? U0[oc + q * fs] : R(0);
-          kav[q] = needk ? ka[oi + q * a.il.fs] : R(0);
+          for (int q = 0; q < 5; ++q) {
+            u0v[q] = f0[q];
+            kav[q] = f1[q];
+          }
+        } else {
+          const R* U0 = static_cast<const R*>(a.U0);
+          const bool need0 = a.update == MFVM_UPD_BASE || a.update == MFVM_UPD_SSP || a.update == MFVM_UPD_ACC6;
+          const bool needk = a.update == MFVM_UPD_ACC6 || a.acc == MFVM_ACC_ADD;
+#pragma unroll
+          for (int q = 0; q < 5; ++q) {
+            u0v[q] = need0 ? U0[oc + q * fs] : R(0);
+            kav[q] = needk ? ka[oi + q * a.il.fs] : R(0);
+          }
         }
 #pragma unroll
         for (int q = 0; q < 5; ++q) {
@@ -986,6 +1028,7 @@ MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl, const S& src)
         const bool fin = FIN == 2 ? a.comp >= MFVM_COMP_FINAL_SET : FIN == 1;
         const bool needd = a.comp == MFVM_COMP_SSP || a.comp == MFVM_COMP_FINAL_SSP;
         const bool needk = a.comp == MFVM_COMP_FINAL_ACC6 || a.acc == MFVM_ACC_ADD;
+        const R* U0 = static_cast<const R*>(a.U0);
         double nw[5];
         R hv[5], lv[5], dvv[5], kav[5];
 #pragma unroll
