@@ -244,6 +244,68 @@ def test_heat_tiles_match_oracle(rec, lim):
             assert same(U2[k].cpu().numpy(), Uo[k]), (steps, k)
 
 
+@pytest.mark.parametrize("rec,lim", [("piecewise_constant", "minmod"), ("muscl", "van_leer")])
+@pytest.mark.parametrize("rho_cp,z_max", [(1.0, 0.5625), (1.5, 1.0)])
+def test_heat_march_chunks_match_oracle(rec, lim, rho_cp, z_max):
+    """A cylindrical grid whose workgroups march several x3 planes (256 x 64
+    x 72: four planes per workgroup, 18 chunks): the right-hand side and 3
+    Euler steps equal the oracle's bit for bit.  rho*cp = 1 with dz = 2^-7
+    takes the specialised power-of-two Euler form with a power-of-two x1 and
+    x3 length (as the cylindrical heat benchmark does); rho*cp = 1.5 the
+    general form."""
+    from oracle import grid as og
+    from oracle import heat as oht
+    from oracle import numerics as on
+    from oracle import solver as osv
+    from mfvm.grid import boundary_conditions as bc
+    from mfvm.grid.geometry import build_grid
+    from mfvm.numerics import time_integrators as ti
+    from mfvm.physics import heat
+    n = (256, 64, 72)
+    gc = grid_config(n, (0.0, 0.0, 0.0), (1.0, 2 * np.pi, z_max))
+    params = {"mass_density": 1.0, "specific_heat_capacity": rho_cp, "thermal_conductivity": 1e-2,
+              "volumetric_heat_source": 0.0}
+    so = {"reconstructor": getattr(on, rec), "slope_limiter": getattr(on, lim),
+          "riemann_solver": None, "diffusive_solver": on.centered}
+    sg = _schemes(rec, lim)
+    go = og.build_grid("cylindrical", gc)
+    grid = build_grid("cylindrical", gc)
+
+    def fresh():
+        rng = np.random.default_rng(4411)
+        Uo = oht.allocate_u(gc)
+        for k in oht.VARS:
+            Uo[k][...] = 1.0 + rng.random(Uo[k].shape)
+        return Uo
+
+    Uo = fresh()
+    ro = oht.compute_rhs(Uo, go, params, so)
+    U = heat.allocate_u(gc)
+    for k in heat.VARS:
+        U[k] = torch.from_numpy(Uo[k])
+    print(f"heat march {n} {rec} rho_cp={rho_cp}: rhs", flush=True)
+    r = heat.compute_rhs(U, grid, params, sg)
+    for k in heat.VARS:
+        assert same(r[k].cpu().numpy(), ro[k]), k
+    sides = {"left": og.neumann, "right": og.neumann, "bottom": og.periodic, "top": og.periodic,
+             "front": og.neumann, "back": og.neumann}
+    gsides = {"left": bc.neumann, "right": bc.neumann, "bottom": bc.periodic, "top": bc.periodic,
+              "front": bc.neumann, "back": bc.neumann}
+    bo = {k: dict(sides) for k in oht.VARS}
+    bg = {k: dict(gsides) for k in heat.VARS}
+    steps = 3
+    dts_o = osv.advance(oht, Uo, go, params, so, on.euler_explicit, bo, 0.4, steps)
+    Ur = fresh()
+    U2 = heat.allocate_u(gc)
+    for k in heat.VARS:
+        U2[k] = torch.from_numpy(Ur[k])
+    print(f"heat march {n} {rec} rho_cp={rho_cp}: {steps} euler steps", flush=True)
+    dts = heat.advance(U2, grid, params, sg, bg, ti.euler_explicit._mfvm_code, 0.4, steps)
+    assert dts.cpu().numpy()[:steps].tolist() == list(dts_o)
+    for k in heat.VARS:
+        assert same(U2[k].cpu().numpy(), Uo[k]), k
+
+
 COORDS = {
     "cartesian": ((0.0, -0.5, 0.25), (1.0, 0.75, 1.5)),
     "cylindrical": ((0.1, 0.0, -1.0), (1.0, 2 * np.pi, 1.0)),
