@@ -8,7 +8,8 @@
 namespace mfvm {
 
 // Defined in ns.hip / bc.hip / heat.hip / maxwell.hip.
-template <class R> hipError_t launch_closure(const mfvm_grid*, const mfvm_ns_params*, const void*, void*, hipStream_t);
+template <class R>
+hipError_t launch_closure(const mfvm_grid*, const mfvm_ns_params*, const void*, void*, hipStream_t, bool with_p = true);
 template <class R> hipError_t launch_stage(const mfvm_grid*, const mfvm_ns_params*, const mfvm_stage*, hipStream_t);
 template <class R> hipError_t launch_cfl(const mfvm_grid*, const mfvm_ns_params*, const void*, const void*, double, double*, double*, hipStream_t);
 hipError_t launch_dt_finalize(const mfvm_grid*, const mfvm_ns_params*, const double*, double, double*, hipStream_t);
@@ -445,6 +446,8 @@ struct NSCtx {
 };
 static int ns_pre(void* c, const void* in, void* C, void* stream) {
   NSCtx* x = static_cast<NSCtx*>(c);
+  if (x->p->dtype == MFVM_F32)  // the float32 stage forms P from U: the closure leaves it out
+    return hip_check(launch_closure<float>(x->g, x->p, in, C, static_cast<hipStream_t>(stream), false), "ns_closure");
   return mfvm_ns_closure(x->g, x->p, in, C, stream);
 }
 static int ns_stage_op(void* c, const void* in, const void* C, const StageIO& io, double* partials, void* stream) {
@@ -551,7 +554,8 @@ int mfvm_ns_comp_advance(const mfvm_grid* g, const mfvm_ns_params* p, const mfvm
   for (int n = 0; n < nsteps; ++n) {
     for (int q = 0; q < ns; ++q) {
       if ((rc = mfvm_apply_bcs(g, 5, bc, MFVM_F32, seq[q].in, seq[q].in, stream))) return rc;
-      if ((rc = mfvm_ns_closure(g, p, seq[q].in, w->C, stream))) return rc;
+      // the closure without P: the float32 stage forms P from U
+      if ((rc = hip_check(launch_closure<float>(g, p, seq[q].in, w->C, s, false), "ns_comp_advance"))) return rc;
       const bool fin = q == ns - 1;
       mfvm_stage st = {};
       st.U = seq[q].in;

@@ -98,8 +98,9 @@ __global__ __launch_bounds__(kEnsBlock) void ens_closure_kernel(const EnsMem* __
     tl.z = w.t / (e.cnx * e.cny);
     Geo geo = e.geo;
     if constexpr (P2) set_pow2(geo);
-    closure_march_body<VISC, UNIF, R>(e.lay, geo, p, static_cast<const R*>(e.buf[sel]), static_cast<R*>(e.C),
-                                      e.ckz, tl);
+    // (float32 members: no P, their stage forms it from U)
+    closure_march_body<VISC, UNIF, R, sizeof(R) == sizeof(double)>(e.lay, geo, p, static_cast<const R*>(e.buf[sel]),
+                                                                   static_cast<R*>(e.C), e.ckz, tl);
     __syncthreads();  // LDS of this tile is read before the next tile writes it
   }
 }

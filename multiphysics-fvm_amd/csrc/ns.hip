@@ -43,12 +43,13 @@ __global__ void ns_closure_p_kernel(Layout lay, NSPar p, const R* __restrict__ U
 
 // P2 (the closure kernel): uniform power-of-two geometry (set_pow2).
 
-template <bool UNIF, class R, bool P2>
+// WP: also write P (the float32 stage passes form P from U themselves).
+template <bool UNIF, class R, bool P2, bool WP>
 __global__ __launch_bounds__(256) void ns_closure_visc_kernel(Layout lay, Geo geo, NSPar p,
                                                               const R* __restrict__ U,
                                                               R* __restrict__ C, int kz, int remap) {
   if constexpr (P2) set_pow2(geo);
-  closure_march_body<true, UNIF, R>(lay, geo, p, U, C, kz, tile_id(remap != 0));
+  closure_march_body<true, UNIF, R, WP>(lay, geo, p, U, C, kz, tile_id(remap != 0));
 }
 
 // The fused stage kernels take SPEC (spec_args) as the split ones do.
@@ -132,9 +133,24 @@ long long ns_nblocks(const mfvm_grid* g, const mfvm_ns_params* p) {
 
 constexpr int CLOSURE_KZ_F64 = 32, CLOSURE_KZ_F32 = 16;
 
+// x3 planes per closure workgroup: the defaults above; MFVM_CLOSURE_KZ
+// overrides both (1 to 64).
+static int closure_kz(bool f32) {
+  static const int env = [] {
+    const char* e = getenv("MFVM_CLOSURE_KZ");
+    const int v = e ? atoi(e) : 0;
+    return v < 0 ? 0 : (v > 64 ? 64 : v);
+  }();
+  return env > 0 ? env : (f32 ? CLOSURE_KZ_F32 : CLOSURE_KZ_F64);
+}
+
+// with_p = false (float32 only): P is left out -- the float32 stage passes
+// form it from U -- and an inviscid closure launches nothing.
 template <class R>
 hipError_t launch_closure(const mfvm_grid* g, const mfvm_ns_params* p, const void* U, void* C,
-                          hipStream_t s) {
+                          hipStream_t s, bool with_p) {
+  if (sizeof(R) == sizeof(double)) with_p = true;
+  if (!with_p && p->mu == 0.0) return hipSuccess;
   const Layout lay = make_layout(g->n[0], g->n[1], g->n[2]);
   const Geo geo = make_geo(g);
   const NSPar q = make_ns_par(p);
@@ -148,18 +164,27 @@ hipError_t launch_closure(const mfvm_grid* g, const mfvm_ns_params* p, const voi
   } else {
     // x3 planes per workgroup: the float32 kernel (no prefetch) measures
     // 13% faster with chunks of 16 than of 32 at 256^3
-    const int kzz = sizeof(R) == sizeof(float) ? CLOSURE_KZ_F32 : CLOSURE_KZ_F64;
+    const int kzz = closure_kz(sizeof(R) == sizeof(float));
     const int kz = g->n[2] < kzz ? g->n[2] : kzz;
     const dim3 grd((g->n[0] + 63) / 64, (g->n[1] + 3) / 4, (g->n[2] + kz - 1) / kz);
     const size_t lds = 3 * (4 + 2) * (64 + 2) * sizeof(R);
     const int remap = xcd_remap_enabled();
-    if (g->uniform && pow2_geo(geo))
-      hipLaunchKernelGGL((ns_closure_visc_kernel<true, R, true>), grd, blk, lds, s, lay, geo, q, u, c, kz, remap);
-    else if (g->uniform)
-      hipLaunchKernelGGL((ns_closure_visc_kernel<true, R, false>), grd, blk, lds, s, lay, geo, q, u, c, kz, remap);
-    else
-      hipLaunchKernelGGL((ns_closure_visc_kernel<false, R, false>), grd, blk, lds, s, lay, geo, q, u, c, kz,
+#define MFVM_CLOSURE_VISC(WPV)                                                                        \
+    if (g->uniform && pow2_geo(geo))                                                                   \
+      hipLaunchKernelGGL((ns_closure_visc_kernel<true, R, true, WPV>), grd, blk, lds, s, lay, geo, q, u, c, kz,  \
+                         remap);                                                                       \
+    else if (g->uniform)                                                                               \
+      hipLaunchKernelGGL((ns_closure_visc_kernel<true, R, false, WPV>), grd, blk, lds, s, lay, geo, q, u, c, kz, \
+                         remap);                                                                       \
+    else                                                                                               \
+      hipLaunchKernelGGL((ns_closure_visc_kernel<false, R, false, WPV>), grd, blk, lds, s, lay, geo, q, u, c, kz, \
                          remap);
+    if (with_p) {
+      MFVM_CLOSURE_VISC(true)
+    } else {
+      MFVM_CLOSURE_VISC(sizeof(R) == sizeof(double))
+    }
+#undef MFVM_CLOSURE_VISC
   }
   t_end(tk, s);
   return hipGetLastError();
@@ -360,8 +385,8 @@ hipError_t launch_dt_append(const mfvm_grid* g, const mfvm_ns_params* p, const d
   return hipGetLastError();
 }
 
-template hipError_t launch_closure<double>(const mfvm_grid*, const mfvm_ns_params*, const void*, void*, hipStream_t);
-template hipError_t launch_closure<float>(const mfvm_grid*, const mfvm_ns_params*, const void*, void*, hipStream_t);
+template hipError_t launch_closure<double>(const mfvm_grid*, const mfvm_ns_params*, const void*, void*, hipStream_t, bool);
+template hipError_t launch_closure<float>(const mfvm_grid*, const mfvm_ns_params*, const void*, void*, hipStream_t, bool);
 template hipError_t launch_stage<double>(const mfvm_grid*, const mfvm_ns_params*, const mfvm_stage*, hipStream_t);
 template hipError_t launch_stage<float>(const mfvm_grid*, const mfvm_ns_params*, const mfvm_stage*, hipStream_t);
 template hipError_t launch_cfl<double>(const mfvm_grid*, const mfvm_ns_params*, const void*, const void*, double, double*, double*, hipStream_t);

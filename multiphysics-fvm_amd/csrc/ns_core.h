@@ -161,15 +161,17 @@ MFVM_DEV void closure_t(const Geo& geo, const NSPar& p, int i, int j, int k, con
 }
 
 // Closure marching in x3.  One thread per (x1, x2) column of a BX x BY tile
-// (the workgroup shape) over a chunk of kz planes.  Each velocity is computed
+// (the workgroup shape) over a chunk of kz planes.  WP = false: P is not
+// written (the float32 stage passes form it from U where they read it).  Each velocity is computed
 // once: the column keeps v at k-1, k, k+1 in registers, and plane k of the
 // tile plus a one-cell ring is exchanged through LDS for the x1 and x2
 // differences.  The ring cells are computed by helper threads; where a ring
 // cell is a face ghost of the domain, its helper also writes its P.  Without
 // viscosity only P is formed (on the same cells).  LDS: 3 (BY+2)(BX+2) R.
-template <bool VISC, bool UNIF, class R>
+template <bool VISC, bool UNIF, class R, bool WP = true>
 MFVM_DEV void closure_march_body(const Layout& lay, const Geo& geo, const NSPar& p,
                                  const R* __restrict__ U, R* __restrict__ C, int kz, TileId tl) {
+  if constexpr (!WP && !VISC) return;
   extern __shared__ double smem_raw[];
   R* vs = reinterpret_cast<R*>(smem_raw);  // [3][BY+2][BX+2]
   const int BX = blockDim.x, BY = blockDim.y;
@@ -205,19 +207,19 @@ MFVM_DEV void closure_march_body(const Layout& lay, const Geo& geo, const NSPar&
   if (own) {
     R rho;
     const long long om = lay.at(i, j, kb - 1), oc = lay.at(i, j, kb);
-    if (VISC || kb - 1 == 0) {
+    if (VISC || (WP && kb - 1 == 0)) {
       load_v(U, om, fs, rho, vm);
-      if (kb - 1 == 0) C[om] = closure_p(p, rho, vm[0], vm[1], vm[2], U[om + 4 * fs]);
+      if (WP && kb - 1 == 0) C[om] = closure_p(p, rho, vm[0], vm[1], vm[2], U[om + 4 * fs]);
     }
     load_v(U, oc, fs, rho, vc);
-    C[oc] = closure_p(p, rho, vc[0], vc[1], vc[2], U[oc + 4 * fs]);
+    if (WP) C[oc] = closure_p(p, rho, vc[0], vc[1], vc[2], U[oc + 4 * fs]);
   }
   // PF (float64): the U values of plane k+1 of the own column (un) and of
   // plane k of the ring cell (uh) are loaded one iteration ahead, so that
   // their latency overlaps the previous plane's work (9% faster).  The
   // float32 kernel runs more waves per SIMD and measures slower with it.
   constexpr bool PF = sizeof(R) == sizeof(double);
-  const bool hload = hv && (VISC || hghost);
+  const bool hload = hv && (VISC || (WP && hghost));
   auto own_load = [&](int k) { return own && (VISC || k + 1 < ke || k + 1 == n3 + 1); };
   R un[5], uh[5];
   if (PF) {
@@ -244,7 +246,7 @@ MFVM_DEV void closure_march_body(const Layout& lay, const Geo& geo, const NSPar&
 #pragma unroll
         for (int a = 0; a < 3; ++a) vs[a * splane + l] = vh[a];
       }
-      if (hghost) C[oh] = closure_p(p, rho, vh[0], vh[1], vh[2], uh[4]);
+      if (WP && hghost) C[oh] = closure_p(p, rho, vh[0], vh[1], vh[2], uh[4]);
       if (PF) load_u5(U, lay.at(hi, hj, k + 1), fs, k + 1 < ke, uh);
     }
     const bool pnext = k + 1 < ke || k + 1 == n3 + 1;
@@ -255,7 +257,7 @@ MFVM_DEV void closure_march_body(const Layout& lay, const Geo& geo, const NSPar&
       vn[0] = qdiv(un[1], rho);
       vn[1] = qdiv(un[2], rho);
       vn[2] = qdiv(un[3], rho);
-      if (pnext) C[on] = closure_p(p, rho, vn[0], vn[1], vn[2], un[4]);
+      if (WP && pnext) C[on] = closure_p(p, rho, vn[0], vn[1], vn[2], un[4]);
       if (PF) load_u5(U, lay.at(i, j, k + 2), fs, k + 1 < ke && own_load(k + 1), un);
     }
     if constexpr (VISC) {
@@ -288,13 +290,27 @@ MFVM_DEV void closure_march_body(const Layout& lay, const Geo& geo, const NSPar&
 // Reads of the stored quantities at ghosted offset o: u(o, f) of the five
 // conserved fields, c(o, f) of the closure (P and the Stokes tensor), both
 // field-major with a field stride of fs elements.
+//
+// In float32 the pressure is not read from the closure but formed from the
+// cell's U, as the closure forms it (navier_stokes.py:94-100; p_from_u): a
+// few FP32 operations instead of 4 B of memory traffic per read.
+template <class R, class S>
+MFVM_DEV R p_from_u(const S& src, long long o) {
+  const R rho = src.u(o, 0);
+  return closure_p(src.par, rho, qdiv(src.u(o, 1), rho), qdiv(src.u(o, 2), rho), qdiv(src.u(o, 3), rho),
+                   src.u(o, 4));
+}
 template <class R>
 struct PtrSrc {
   const R* __restrict__ U;
   const R* __restrict__ C;
   long long fs;
+  NSPar par;
   MFVM_DEV R u(long long o, int f) const { return U[o + f * fs]; }
-  MFVM_DEV R c(long long o, int f) const { return C[o + f * fs]; }
+  MFVM_DEV R c(long long o, int f) const {
+    if (sizeof(R) == sizeof(float) && f == C_P) return p_from_u<R>(*this, o);
+    return C[o + f * fs];
+  }
 };
 
 // The same reads through buffer resources: one 32-bit byte offset per cell
@@ -306,6 +322,7 @@ template <class R>
 struct BufSrc {
   __amdgpu_buffer_rsrc_t U, C;
   unsigned fsb;  // field stride in bytes
+  NSPar par;
   MFVM_DEV R ld(__amdgpu_buffer_rsrc_t r, long long o, int f) const {
     const int vo = (int)((unsigned)o * (unsigned)sizeof(R));
     const int so = (int)((unsigned)f * fsb);
@@ -315,7 +332,10 @@ struct BufSrc {
       return __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0));
   }
   MFVM_DEV R u(long long o, int f) const { return ld(U, o, f); }
-  MFVM_DEV R c(long long o, int f) const { return ld(C, o, f); }
+  MFVM_DEV R c(long long o, int f) const {
+    if (sizeof(R) == sizeof(float) && f == C_P) return p_from_u<R>(*this, o);
+    return ld(C, o, f);
+  }
 };
 
 // Stage kernels specialised by SPEC (host-checked): 1 -- uniform power-of-
@@ -329,7 +349,7 @@ MFVM_DEV void spec_args(StageArgs& a, int spec) {
 
 template <class R>
 MFVM_DEV PtrSrc<R> ptr_src(const StageArgs& a) {
-  return PtrSrc<R>{static_cast<const R*>(a.U), static_cast<const R*>(a.C), a.lay.fs};
+  return PtrSrc<R>{static_cast<const R*>(a.U), static_cast<const R*>(a.C), a.lay.fs, a.p};
 }
 
 // Buffer resource over [p, p + 4 GiB): raw (stride 0), the gfx9 data-format
@@ -340,7 +360,7 @@ MFVM_DEV __amdgpu_buffer_rsrc_t buf_rsrc(const void* p) {
 
 template <class R>
 MFVM_DEV BufSrc<R> buf_src(const StageArgs& a) {
-  return BufSrc<R>{buf_rsrc(a.U), buf_rsrc(a.C), (unsigned)(a.lay.fs * (long long)sizeof(R))};
+  return BufSrc<R>{buf_rsrc(a.U), buf_rsrc(a.C), (unsigned)(a.lay.fs * (long long)sizeof(R)), a.p};
 }
 
 template <int D, class R, class S>
@@ -887,9 +907,12 @@ MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl, const S& src)
 #pragma unroll
       for (int q = 0; q < 5; ++q) fxy[q] = static_cast<const R*>(a.xy)[oi + q * a.il.fs];
     };
+    // (U0 == U, as in the first stage of a step: U0 of cell m-1 is the
+    // window's zp, read already)
+    const bool u0_own = a.U0 == a.U;
     auto base_reads = [&](int k) {
       const long long oc = a.lay.at(ic, jc, k), oi = a.il.at(ic, jc, k);
-      const bool need0 = a.update == MFVM_UPD_BASE || a.update == MFVM_UPD_SSP || a.update == MFVM_UPD_ACC6;
+      const bool need0 = !u0_own && (a.update == MFVM_UPD_BASE || a.update == MFVM_UPD_SSP || a.update == MFVM_UPD_ACC6);
       const bool needk = a.update == MFVM_UPD_ACC6 || a.acc == MFVM_ACC_ADD;
 #pragma unroll
       for (int q = 0; q < 5; ++q) {
@@ -979,7 +1002,7 @@ MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl, const S& src)
         if constexpr (EARLY) {
 #pragma unroll
           for (int q = 0; q < 5; ++q) {
-            u0v[q] = f0[q];
+            u0v[q] = u0_own ? zp[q] : f0[q];
             kav[q] = f1[q];
           }
         } else {
@@ -988,7 +1011,7 @@ MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl, const S& src)
           const bool needk = a.update == MFVM_UPD_ACC6 || a.acc == MFVM_ACC_ADD;
 #pragma unroll
           for (int q = 0; q < 5; ++q) {
-            u0v[q] = need0 ? U0[oc + q * fs] : R(0);
+            u0v[q] = u0_own ? zp[q] : (need0 ? U0[oc + q * fs] : R(0));
             kav[q] = needk ? ka[oi + q * a.il.fs] : R(0);
           }
         }
@@ -1034,7 +1057,7 @@ MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl, const S& src)
 #pragma unroll
         for (int q = 0; q < 5; ++q) {
           const long long og = oc + q * fs, ii = oi + q * a.il.fs;
-          hv[q] = U0[og];
+          hv[q] = u0_own ? zp[q] : U0[og];
           lv[q] = fin ? lo[og] : R(0);
           dvv[q] = needd ? dv[ii] : R(0);
           kav[q] = needk ? ka[ii] : R(0);
