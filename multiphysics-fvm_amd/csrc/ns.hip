@@ -191,23 +191,28 @@ hipError_t launch_closure(const mfvm_grid* g, const mfvm_ns_params* p, const voi
 }
 
 // Rows per chunk of the row-march x1/x2 pass: 16 (FP64 and FP32 both 1-6%
-// faster than 24 or 32 at 256^3); MFVM_XYROW_ROWS overrides (2 to XR_ROWS).
-static int xyrow_rows() {
-  static const int r = [] {
+// faster than 24 or 32 at 256^3), halved (down to 2) while the pass would
+// have fewer than 4096 wave tasks -- a 2D 256^2 grid has 64 of 16 rows, which
+// leave most CUs idle; MFVM_XYROW_ROWS overrides (2 to XR_ROWS).
+static int xyrow_rows(long long strips, int n2, long long planes) {
+  static const int env = [] {
     const char* e = getenv("MFVM_XYROW_ROWS");
-    const int v = e ? atoi(e) : 16;
-    return v < 2 ? 2 : (v > XR_ROWS ? XR_ROWS : v);
+    const int v = e ? atoi(e) : 0;
+    return v <= 0 ? 0 : (v < 2 ? 2 : (v > XR_ROWS ? XR_ROWS : v));
   }();
+  if (env) return env;
+  int r = 16;
+  while (r > 2 && strips * ((n2 + r - 1) / r) * planes < 4096) r /= 2;
   return r;
 }
 
 template <int LIM, int RS, bool UNIF, class R, int SPEC>
 static void xyrow_launch(const StageArgs& a0, bool buf, hipStream_t s) {
   StageArgs a = a0;
-  a.kz = xyrow_rows();
   const long long ns = (a.lay.n1 + XR_W - 1) / XR_W;
-  const long long nc = (a.lay.n2 + a.kz - 1) / a.kz;
   const int m1 = a.xm1 > 0 ? a.xm1 : a.lay.n3 + 1;
+  a.kz = xyrow_rows(ns, a.lay.n2, m1 - a.xm0);
+  const long long nc = (a.lay.n2 + a.kz - 1) / a.kz;
   const long long ntask = ns * nc * (m1 - a.xm0);
   const dim3 g((unsigned)((ntask + XR_WAVES - 1) / XR_WAVES)), b(XR_W * XR_WAVES);
   const size_t lds = (size_t)XR_WAVES * XR_LDS_ELEMS * sizeof(R);

@@ -82,9 +82,10 @@ def compute_dt(U, params, grid, cfl):
     return float(dt[0].item())
 
 
-# 2D grids split from this many cells on (smaller ones are launch-bound and
-# lose the extra launch); MFVM_SPLIT_2D=0/1 forces either way.
-SPLIT_2D_MIN_CELLS = 256 * 256
+# 2D grids split from this many cells on: below it the fused stage is faster
+# (Kelvin-Helmholtz runs to t = 2 measured 324 ms fused against 379 split at
+# 256^2, 1918 against 1467 at 512^2); MFVM_SPLIT_2D=0/1 forces either way.
+SPLIT_2D_MIN_CELLS = 512 * 512
 
 
 def split_2d(U):
