@@ -261,8 +261,16 @@ def ensemble(args, torch):
         best = ens.autotune(0.4, steps=3)
         tune_s = time.perf_counter() - t0
         ms = _time(lambda: ens.advance(0.4, steps), torch)
+        # the autotuner's timed trials of the persistent launch (k workgroups
+        # per CU over the whole work list) against one workgroup per tile,
+        # with the other options as chosen
+        pers = {}
+        for o, t in ens.tuning_log:
+            if all(o.get(k) == best[k] for k in ("split", "order", "remap")) and o["tiles"] == best["tiles"]:
+                pers[f"persist={o['persist']}"] = round(t, 3)
         out[leg] = {"ms": round(ms, 2), "Mcell_updates_per_s": round(cells * steps / ms * 1e-3, 1),
                     "autotune_s": round(tune_s, 2), "trials": len(ens.tuning_log),
+                    "autotune_ms_per_step": pers,
                     "options": {"split": best["split"], "order": best["order"], "persist": best["persist"],
                                 "remap": best["remap"], "graph": best["graph"],
                                 "tiles": {"x".join(map(str, k)): list(v) for k, v in best["tiles"].items()}}}

@@ -425,6 +425,8 @@ typedef struct mfvm_closure_work {
   double* dt_cur;               /* 1 double */
   double* t_cur;                /* 1 double: time reached (from 0 at the first call's start) */
   int* step;                    /* 1 int: dts written so far */
+  int32_t layered;              /* 0: the network and the correction of a stage in one launch
+                                   (a1, a2, y unused, may be NULL); 1: one launch per layer */
 } mfvm_closure_work;
 
 /* nsteps rk3_ssp steps (time_integrators.py:143-159 with

@@ -24,11 +24,14 @@ template <class R> hipError_t maxwell_launch_stage(const mfvm_grid*, const mfvm_
 hipError_t maxwell_launch_dt(const mfvm_grid*, const mfvm_maxwell_params*, double, double*, hipStream_t);
 template <class R> hipError_t launch_cfl_partials(const mfvm_grid*, const mfvm_ns_params*, const void*, const void*, double*, hipStream_t);
 hipError_t launch_dt_append(const mfvm_grid*, const mfvm_ns_params*, const double*, double, double, double*, double*,
-                            double*, int*, hipStream_t);
+                            double*, int*, int*, int, hipStream_t);
 // Defined in closure.hip.
 hipError_t cnn_conv(int mode, int cin, int cout, int batch, int n1, int n2, const void* x, const void* w,
                     const float* b, const void* mask, void* y, hipStream_t s);
 bool cnn_shape_ok(int mode, int cin, int cout);
+hipError_t closure_net_apply(const mfvm_grid* g, double gamma, const void* U, const void* out, const float* shift,
+                             const float* scale, const void* const wf[3], const float* const b[3], const float* osc,
+                             double coef, const double* dt, void* T, int* flag, float* yg, hipStream_t s);
 hipError_t cnn_conv_features(const mfvm_grid* g, double gamma, const void* U, const float* shift, const float* scale,
                              const void* w, const float* b, void* y, hipStream_t s);
 hipError_t cnn_wgrad(int cin, int cout, int batch, int n1, int n2, const void* x, const void* gy, float* gw,
@@ -722,7 +725,8 @@ static int closure_step(const mfvm_grid* g, const mfvm_ns_params* p, const mfvm_
                     {w->S2, w->U, MFVM_UPD_SSP, 1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
   const int n1 = g->n[0], n2 = g->n[1];
   const int32_t* bc = &p->bc[0][0];
-  hipError_t e = fill_int(cw->flag, 3, 0, s);
+  // (the stage flags start at 0: the previous step's dt append, or the set-up, cleared them)
+  hipError_t e = hipSuccess;
   for (int q = 0; q < 3 && e == hipSuccess; ++q) {
     e = launch_bcs<double>(g, 5, bc, seq[q].in, seq[q].in, s);
     if (e == hipSuccess) e = launch_closure<double>(g, p, seq[q].in, w->C, s);
@@ -740,18 +744,30 @@ static int closure_step(const mfvm_grid* g, const mfvm_ns_params* p, const mfvm_
     st.xy = w->xy;  // split stages where the caller provides the scratch (same results)
     if (e == hipSuccess) e = launch_stage<double>(g, p, &st, s);
     if (!net) continue;
-    // features of the stage input and the first layer in one launch
-    if (e == hipSuccess)
-      e = cnn_conv_features(g, p->gamma, seq[q].in, net->in_shift, net->in_scale, net->wf[0], net->b[0], cw->a1, s);
-    if (e == hipSuccess) e = cnn_conv(MFVM_CNN_FWD_RELU, 32, 32, 1, n1, n2, cw->a1, net->wf[1], net->b[1], nullptr, cw->a2, s);
-    if (e == hipSuccess) e = cnn_conv(MFVM_CNN_FWD_LINEAR, 32, 8, 1, n1, n2, cw->a2, net->wf[2], net->b[2], nullptr, cw->y, s);
-    if (e == hipSuccess)
-      e = closure_apply<double>(g, p->gamma, seq[q].out, cw->y, net->out_scale, seq[q].coef, cw->dt_cur, cw->T,
-                                cw->flag + q, s);
+    if (!cw->layered) {
+      // features, the three layers and the corrected state in one launch
+      const void* wf[3] = {net->wf[0], net->wf[1], net->wf[2]};
+      const float* b[3] = {net->b[0], net->b[1], net->b[2]};
+      if (e == hipSuccess)
+        e = closure_net_apply(g, p->gamma, seq[q].in, seq[q].out, net->in_shift, net->in_scale, wf, b, net->out_scale,
+                              seq[q].coef, cw->dt_cur, cw->T, cw->flag + q, nullptr, s);
+    } else {
+      // features of the stage input and the first layer in one launch, then one per layer
+      if (e == hipSuccess)
+        e = cnn_conv_features(g, p->gamma, seq[q].in, net->in_shift, net->in_scale, net->wf[0], net->b[0], cw->a1, s);
+      if (e == hipSuccess)
+        e = cnn_conv(MFVM_CNN_FWD_RELU, 32, 32, 1, n1, n2, cw->a1, net->wf[1], net->b[1], nullptr, cw->a2, s);
+      if (e == hipSuccess)
+        e = cnn_conv(MFVM_CNN_FWD_LINEAR, 32, 8, 1, n1, n2, cw->a2, net->wf[2], net->b[2], nullptr, cw->y, s);
+      if (e == hipSuccess)
+        e = closure_apply<double>(g, p->gamma, seq[q].out, cw->y, net->out_scale, seq[q].coef, cw->dt_cur, cw->T,
+                                  cw->flag + q, s);
+    }
     if (e == hipSuccess) e = closure_select<double>(g, cw->T, seq[q].out, cw->flag + q, cw->counts + q, s);
   }
   if (e == hipSuccess) e = launch_cfl_partials<double>(g, p, w->U, nullptr, w->partials, s);
-  if (e == hipSuccess) e = launch_dt_append(g, p, w->partials, cfl, t_end, w->dts, cw->dt_cur, cw->t_cur, cw->step, s);
+  if (e == hipSuccess)
+    e = launch_dt_append(g, p, w->partials, cfl, t_end, w->dts, cw->dt_cur, cw->t_cur, cw->step, cw->flag, 3, s);
   return hip_check(e, "closure_advance");
 }
 
@@ -775,7 +791,8 @@ int mfvm_closure_advance(const mfvm_grid* g, const mfvm_ns_params* p, const mfvm
     for (int l = 0; l < 3; ++l)
       if (!net->wf[l] || !net->b[l]) return fail(MFVM_E_ARG, "closure_advance: NULL network weights");
     if (!net->in_shift || !net->in_scale || !net->out_scale) return fail(MFVM_E_ARG, "closure_advance: NULL scales");
-    if (!cw->a1 || !cw->a2 || !cw->y || !cw->T) return fail(MFVM_E_ARG, "closure_advance: NULL work buffer");
+    if (!cw->T || (cw->layered && (!cw->a1 || !cw->a2 || !cw->y)))
+      return fail(MFVM_E_ARG, "closure_advance: NULL work buffer");
   }
   if (t_end != t_end) return fail(MFVM_E_ARG, "closure_advance: t_end is NaN");
   if (nsteps < 0 || graph < 0) return fail(MFVM_E_ARG, "nsteps and graph must be >= 0");
@@ -785,7 +802,8 @@ int mfvm_closure_advance(const mfvm_grid* g, const mfvm_ns_params* p, const mfvm
   if (e == hipSuccess) e = fill_int(cw->counts, 3, 0, s);
   if (e == hipSuccess) e = hipMemsetAsync(cw->t_cur, 0, sizeof(double), s);  // not captured
   if (e == hipSuccess) e = launch_cfl_partials<double>(g, p, w->U, nullptr, w->partials, s);
-  if (e == hipSuccess) e = launch_dt_append(g, p, w->partials, cfl, t_end, w->dts, cw->dt_cur, cw->t_cur, cw->step, s);
+  if (e == hipSuccess)
+    e = launch_dt_append(g, p, w->partials, cfl, t_end, w->dts, cw->dt_cur, cw->t_cur, cw->step, cw->flag, 3, s);
   if ((rc = hip_check(e, "closure_advance"))) return rc;
   int done = 0;
   if (graph > 0) {

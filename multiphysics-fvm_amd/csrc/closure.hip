@@ -400,6 +400,178 @@ __global__ void closure_select_kernel(Layout lay, const R* __restrict__ T, R* __
   }
 }
 
+// ---------------------------------------------------------------------------
+// Fused inference: features, the three convolutions and the corrected stage
+// state of one stage in one launch.
+// ---------------------------------------------------------------------------
+// One workgroup (4 waves) per CT x CT tile of cells (CT = 16, or 8 on grids
+// below 256^2, whose 16 x 16 tiles would leave most CUs idle).  The tile's corrected
+// state needs the network output y at its cells and one more column / row
+// (the faces i+1/2, j+1/2 of its last cells), so layer 3 is evaluated on
+// (CT+1)^2 pixels, layer 2 on (CT+3)^2, layer 1 on (CT+5)^2 and the features
+// on (CT+7)^2 (periodic), each layer's output held in LDS (channels-last,
+// BF16; layer 3 in FP32).  The convolutions are implicit GEMMs on the gfx950
+// v_mfma_f32_16x16x32_bf16 (16 output channels x 16 pixels per instruction,
+// K = 32 = one tap of 32 input channels, or 8 taps of the 4 features; lane l
+// holds A[l%16][8(l/16) .. +7], B[8(l/16) .. +7][l%16], D[4(l/16) .. +3][l%16]).
+// Each pixel's activations and y are those of the layer kernels above (BF16
+// rounding after the ReLU, FP32 accumulation); only the order of the FP32
+// sums inside a 32-wide K block differs.  The corrected state and the
+// positivity flag follow closure_apply_kernel.
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+// CT: tile width (16, or 8 where 16 would give fewer than 256 workgroups)
+template <int CT> struct NetTile {
+  static constexpr int W0 = CT + 7, W1 = CT + 5, W2 = CT + 3, W3 = CT + 1;  // region widths
+  // LDS: the regions in BF16 elements, each starting 16-byte aligned (the
+  // layer reads are 16-byte loads); layer 3 (FP32) last
+  static constexpr int NF0 = (W0 * W0 * 4 + 7) / 8 * 8, NY1 = W1 * W1 * 32, NY2 = W2 * W2 * 32;
+  static constexpr size_t lds() { return (size_t)(NF0 + NY1 + NY2) * 2 + (size_t)W3 * W3 * 8 * 4; }
+};
+
+MFVM_DEV bf16x8 ld_bf16x8(const uint16_t* p) { return __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(p)); }
+MFVM_DEV bf16x8 two_bf16x4(const uint16_t* p, bool okp, const uint16_t* q, bool okq) {
+  const uint2 a = okp ? *reinterpret_cast<const uint2*>(p) : uint2{0u, 0u};
+  const uint2 b = okq ? *reinterpret_cast<const uint2*>(q) : uint2{0u, 0u};
+  return __builtin_bit_cast(bf16x8, uint4{a.x, a.y, b.x, b.y});
+}
+
+// One 3x3 layer over an LDS region: input win x win pixels of CIN channels,
+// output (win-2)^2 pixels of COUT channels (ReLU -> BF16, or linear FP32).
+// The wave's weight fragments (every K step and output block) are read once
+// into registers, then reused for each 16-pixel tile the wave takes.
+template <int CIN, int COUT, bool RELU>
+MFVM_DEV void net_layer(const uint16_t* __restrict__ xin, int win, const uint16_t* __restrict__ w,
+                        const float* __restrict__ bias, void* yout) {
+  constexpr int COUTP = Round16<COUT>::v, NMB = COUTP / 16;
+  constexpr int NKS = CIN == 4 ? 2 : kTap;  // K steps of 32
+  const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15, wave = threadIdx.x >> 6;
+  const int wout = win - 2, npix = wout * wout, ntile = (npix + 15) / 16;
+  bf16x8 av[NKS][NMB];
+#pragma unroll
+  for (int s = 0; s < NKS; ++s) {
+#pragma unroll
+    for (int mb = 0; mb < NMB; ++mb) {
+      const int co = mb * 16 + c;
+      if constexpr (CIN == 4) {
+        // k = 32 s + 8 g + e: taps 8 s + 2 g and 8 s + 2 g + 1, 4 features each
+        const int t0 = 8 * s + 2 * g, t1 = t0 + 1;
+        av[s][mb] = two_bf16x4(w + ((long long)min(t0, 8) * COUTP + co) * 4, t0 < kTap,
+                               w + ((long long)min(t1, 8) * COUTP + co) * 4, t1 < kTap);
+      } else {
+        // k = 32 s + 8 g + e: tap s, input channels 8 g .. 8 g + 7
+        av[s][mb] = ld_bf16x8(w + ((long long)s * COUTP + co) * CIN + 8 * g);
+      }
+    }
+  }
+  float bs[NMB][4];
+#pragma unroll
+  for (int mb = 0; mb < NMB; ++mb)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) bs[mb][r] = mb * 16 + 4 * g + r < COUT ? bias[mb * 16 + 4 * g + r] : 0.f;
+  for (int t = wave; t < ntile; t += 4) {
+    const int p = min(16 * t + c, npix - 1);  // pixels past the region repeat the last (not stored)
+    const int ox = p % wout, oy = p / wout;
+    f32x4 acc[NMB];
+#pragma unroll
+    for (int mb = 0; mb < NMB; ++mb) acc[mb] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < NKS; ++s) {
+      bf16x8 bv;
+      if constexpr (CIN == 4) {
+        const int t0 = 8 * s + 2 * g, t1 = t0 + 1;
+        const uint16_t* x0 = xin + ((oy + min(t0, 8) / 3) * win + ox + min(t0, 8) % 3) * 4;
+        const uint16_t* x1 = xin + ((oy + min(t1, 8) / 3) * win + ox + min(t1, 8) % 3) * 4;
+        bv = two_bf16x4(x0, t0 < kTap, x1, t1 < kTap);
+      } else {
+        bv = ld_bf16x8(xin + ((oy + s / 3) * win + ox + s % 3) * CIN + 8 * g);
+      }
+#pragma unroll
+      for (int mb = 0; mb < NMB; ++mb) acc[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[s][mb], bv, acc[mb], 0, 0, 0);
+    }
+    if (16 * t + c >= npix) continue;
+#pragma unroll
+    for (int mb = 0; mb < NMB; ++mb) {
+      const int co0 = mb * 16 + 4 * g;
+      if (co0 >= COUT) continue;
+      if constexpr (RELU) {
+        bf16x4 o;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) o[r] = (short)f2bf(fmaxf(acc[mb][r] + bs[mb][r], 0.f));
+        *reinterpret_cast<bf16x4*>(static_cast<uint16_t*>(yout) + p * COUT + co0) = o;
+      } else {
+        float4 o;
+        o.x = acc[mb][0] + bs[mb][0];
+        o.y = acc[mb][1] + bs[mb][1];
+        o.z = acc[mb][2] + bs[mb][2];
+        o.w = acc[mb][3] + bs[mb][3];
+        *reinterpret_cast<float4*>(static_cast<float*>(yout) + p * COUT + co0) = o;
+      }
+    }
+  }
+}
+
+// yg != NULL: the network output y of the tile's cells is also written there
+// ([n2][n1][8] FP32, as cnn_conv's), for checking against the layer kernels.
+template <int CT>
+__global__ __launch_bounds__(256) void closure_net_apply_kernel(
+    Layout lay, double gm1, const double* __restrict__ U, const double* __restrict__ out, const float* __restrict__ shift,
+    const float* __restrict__ scale, const uint16_t* __restrict__ w0, const float* __restrict__ b0,
+    const uint16_t* __restrict__ w1, const float* __restrict__ b1, const uint16_t* __restrict__ w2,
+    const float* __restrict__ b2, const float* __restrict__ osc, float inv1, float inv2, double coef,
+    const double* __restrict__ dt, double* __restrict__ T, int* __restrict__ flag, float* __restrict__ yg) {
+  using NT = NetTile<CT>;
+  constexpr int CW0 = NT::W0, CW1 = NT::W1, CW2 = NT::W2, CW3 = NT::W3;
+  extern __shared__ double smem_raw[];
+  uint16_t* F0 = reinterpret_cast<uint16_t*>(smem_raw);  // [CW0^2][4]
+  uint16_t* Y1 = F0 + NT::NF0;                           // [CW1^2][32]
+  uint16_t* Y2 = Y1 + NT::NY1;                           // [CW2^2][32]
+  float* Y3 = reinterpret_cast<float*>(Y2 + NT::NY2);    // [CW3^2][8]
+  const int n1 = lay.n1, n2 = lay.n2;
+  const int i0 = blockIdx.x * CT, j0 = blockIdx.y * CT;  // 0-based interior
+  // features of the stage input (cnn_feat4), periodic
+  for (int e = threadIdx.x; e < CW0 * CW0; e += blockDim.x) {
+    const int x = wrap(i0 - 3 + e % CW0, n1), y = wrap(j0 - 3 + e / CW0, n2);
+    *reinterpret_cast<bf16x4*>(F0 + e * 4) = cnn_feat4<double>(lay.fs, gm1, U, lay.at(x + 1, y + 1, 1), shift, scale);
+  }
+  __syncthreads();
+  net_layer<4, 32, true>(F0, CW0, w0, b0, Y1);
+  __syncthreads();
+  net_layer<32, 32, true>(Y1, CW1, w1, b1, Y2);
+  __syncthreads();
+  net_layer<32, 8, false>(Y2, CW2, w2, b2, Y3);
+  __syncthreads();
+  // corrected state of the tile's cells (closure_kc, closure_apply_kernel)
+  const int tx = threadIdx.x % CT, ty = threadIdx.x / CT;
+  const int i = i0 + tx, j = j0 + ty;
+  int bad = 0;
+  if (ty < CT && i < n1 && j < n2) {
+    const float* y0 = Y3 + (ty * CW3 + tx) * 8;
+    const float* yx = y0 + 8;
+    const float* yy = y0 + CW3 * 8;
+    float kc[4];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) kc[f] = -(((yx[f] - y0[f]) * inv1 + (yy[4 + f] - y0[4 + f]) * inv2) * osc[f]);
+    const double cd = coef * *dt;
+    const long long o = lay.at(i + 1, j + 1, 1);
+    const int fld[4] = {0, 1, 2, 4};
+    double v[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) v[q] = out[o + q * lay.fs];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) v[fld[f]] += cd * double(kc[f]);
+    const double rho = v[0];
+    const double p = gm1 * (v[4] - 0.5 * (v[1] * v[1] + v[2] * v[2] + v[3] * v[3]) / rho);
+    if (!(rho > 0.0) || !(p > 0.0)) bad = 1;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) T[o + q * lay.fs] = v[q];
+    if (yg) {
+#pragma unroll
+      for (int f = 0; f < 8; ++f) yg[((long long)j * n1 + i) * 8 + f] = y0[f];
+    }
+  }
+  if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
+}
+
 __global__ void fill_int_kernel(int* p, long long n, int v) {
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
     p[i] = v;
@@ -539,6 +711,28 @@ hipError_t closure_select(const mfvm_grid* g, const void* T, void* out, const in
 template hipError_t closure_apply<double>(const mfvm_grid*, double, const void*, const float*, const float*, double,
                                           const double*, void*, int*, hipStream_t);
 template hipError_t closure_select<double>(const mfvm_grid*, const void*, void*, const int*, int*, hipStream_t);
+
+// Features, the three layers and the corrected stage state T of one stage in
+// one launch (closure_net_apply_kernel); the tile grid covers the 2D grid.
+hipError_t closure_net_apply(const mfvm_grid* g, double gamma, const void* U, const void* out, const float* shift,
+                             const float* scale, const void* const wf[3], const float* const b[3], const float* osc,
+                             double coef, const double* dt, void* T, int* flag, float* yg, hipStream_t s) {
+  const Layout lay = make_layout(g->n[0], g->n[1], g->n[2]);
+  const float inv1 = (float)(1.0 / g->lval[0]), inv2 = (float)(1.0 / g->lval[1]);
+  const bool big = (long long)((g->n[0] + 15) / 16) * ((g->n[1] + 15) / 16) >= 256;
+#define MFVM_NET_LAUNCH(CTV)                                                                                    \
+  hipLaunchKernelGGL((closure_net_apply_kernel<CTV>), dim3((g->n[0] + CTV - 1) / CTV, (g->n[1] + CTV - 1) / CTV), \
+                     dim3(256), NetTile<CTV>::lds(), s, lay, gamma - 1.0, static_cast<const double*>(U),         \
+                     static_cast<const double*>(out), shift, scale, static_cast<const uint16_t*>(wf[0]), b[0],   \
+                     static_cast<const uint16_t*>(wf[1]), b[1], static_cast<const uint16_t*>(wf[2]), b[2], osc,  \
+                     inv1, inv2, coef, dt, static_cast<double*>(T), flag, yg)
+  if (big)
+    MFVM_NET_LAUNCH(16);
+  else
+    MFVM_NET_LAUNCH(8);
+#undef MFVM_NET_LAUNCH
+  return hipGetLastError();
+}
 
 hipError_t fill_int(int* p, long long n, int v, hipStream_t s) {
   hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(256), 0, s, p, n, v);

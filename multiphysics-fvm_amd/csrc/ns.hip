@@ -360,9 +360,11 @@ hipError_t launch_cfl(const mfvm_grid* g, const mfvm_ns_params* p, const void* U
 
 // dt of the next step appended to a history: dt = min(cfl dt, t_end - t),
 // hist[*step] = dt, *dt_cur = dt, *t_cur += dt, ++*step (replayable from a
-// captured graph; t_end = +inf: no clipping).
+// captured graph; t_end = +inf: no clipping).  clear != NULL: clear[0..nclear)
+// are set to 0 as well (per-step flags of the next step, one launch fewer).
 __global__ void ns_dt_append_kernel(const double* partials, long long nblk, double cfl, double smax, double t_end,
-                                    double* hist, double* dt_cur, double* t_cur, int* step) {
+                                    double* hist, double* dt_cur, double* t_cur, int* step, int* clear,
+                                    int nclear) {
   double dt = dt_reduce_block(partials, nblk, cfl, smax);
   if (threadIdx.x == 0) {
     const double t = *t_cur;
@@ -373,6 +375,7 @@ __global__ void ns_dt_append_kernel(const double* partials, long long nblk, doub
     *t_cur = t + dt;
     *step = n + 1;
   }
+  if (clear && (int)threadIdx.x < nclear) clear[threadIdx.x] = 0;
 }
 
 hipError_t launch_dt_finalize(const mfvm_grid* g, const mfvm_ns_params* p, const double* partials,
@@ -384,9 +387,11 @@ hipError_t launch_dt_finalize(const mfvm_grid* g, const mfvm_ns_params* p, const
 }
 
 hipError_t launch_dt_append(const mfvm_grid* g, const mfvm_ns_params* p, const double* partials, double cfl,
-                            double t_end, double* hist, double* dt_cur, double* t_cur, int* step, hipStream_t s) {
+                            double t_end, double* hist, double* dt_cur, double* t_cur, int* step, int* clear,
+                            int nclear, hipStream_t s) {
+  if (nclear > 1024) return hipErrorInvalidValue;
   hipLaunchKernelGGL(ns_dt_append_kernel, dim3(1), dim3(1024), 0, s, partials, ns_nblocks(g, p), cfl, g->smax,
-                     t_end, hist, dt_cur, t_cur, step);
+                     t_end, hist, dt_cur, t_cur, step, clear, nclear);
   return hipGetLastError();
 }
 

@@ -113,7 +113,7 @@ class ClosureWork(ctypes.Structure):
     _fields_ = [("x", ctypes.c_void_p), ("a1", ctypes.c_void_p), ("a2", ctypes.c_void_p),
                 ("y", ctypes.c_void_p), ("T", ctypes.c_void_p), ("flag", ctypes.c_void_p),
                 ("counts", ctypes.c_void_p), ("dt_cur", ctypes.c_void_p), ("t_cur", ctypes.c_void_p),
-                ("step", ctypes.c_void_p)]
+                ("step", ctypes.c_void_p), ("layered", ctypes.c_int32)]
 
 
 # (name, restype, argtypes) of every exported entry point.

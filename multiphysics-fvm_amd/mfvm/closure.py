@@ -189,9 +189,10 @@ class FluxCorrectionCNN:
 # Time stepping with the closure
 # ----------------------------------------------------------------------------
 
-def _closure_work(U):
+def _closure_work(U, layered=False):
     n1, n2, _ = U.n
     w = _lib.ClosureWork()
+    w.layered = int(bool(layered))
     w.a1 = _int16(U, "cnn_a1", (n2, n1, 32)).data_ptr()
     w.a2 = _int16(U, "cnn_a2", (n2, n1, 32)).data_ptr()
     w.y = _typed(U, "cnn_y", (n2, n1, 8), torch.float32).data_ptr()
@@ -219,9 +220,11 @@ def _int16(U, key, shape):
 _SIDE = {}
 
 
-def advance(U, grid, params, space_schemes, bcs, net, cfl, nsteps, t_end=math.inf, graph=0):
+def advance(U, grid, params, space_schemes, bcs, net, cfl, nsteps, t_end=math.inf, graph=0, layered=False):
     """``nsteps`` SSPRK3 steps with the learned correction (net=None: the
-    plain scheme).  dt follows the CFL rule, clipped at ``t_end``.  Returns
+    plain scheme).  dt follows the CFL rule, clipped at ``t_end``.  The
+    network and the correction of a stage run as one fused launch
+    (layered=True: one launch per layer, the training kernels).  Returns
     (dts [nsteps+1] device tensor, fallback counts per stage [3] device tensor,
     time reached as a device scalar)."""
     c.require_state(U, ns.VARS)
@@ -232,7 +235,7 @@ def advance(U, grid, params, space_schemes, bcs, net, cfl, nsteps, t_end=math.in
     w.partials = U.scratch64("partials", max(int(nparts), 1)).data_ptr()
     dts = torch.zeros(nsteps + 1, dtype=torch.float64, device=U.device)
     w.dts = dts.data_ptr()
-    cw = _closure_work(U)
+    cw = _closure_work(U, layered)
     cnn = ctypes.byref(net.c_struct()) if net is not None else None
     cur = torch.cuda.current_stream()
     if graph > 0 and cur.cuda_stream == 0:

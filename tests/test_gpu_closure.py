@@ -230,6 +230,30 @@ def test_small_correction_changes_the_solution_without_fallback():
     assert 0 < float(d) < 1e-2
 
 
+@pytest.mark.parametrize("n,graph", [(64, 0), (128, 4)])
+def test_fused_inference_matches_the_layer_kernels(n, graph):
+    """The fused stage inference (features, three layers and the corrected
+    state in one launch, gfx950 K=32 MFMA) against the same network run one
+    launch per layer (K=16 MFMA): the same BF16 activations up to the order of
+    the FP32 sums inside a 32-wide block, so after 8 steps the states agree to
+    1e-9 and no stage falls back in either."""
+    from mfvm import closure
+    ns, U, grid, params, schemes, bcs = _kh(n, seed=2)
+    V = ns.allocate_u({"nx1": n, "nx2": n, "nx3": 1})
+    V.buf.copy_(U.buf)
+    net = _net(seed=7)
+    net.out_scale.fill_(2e-3)
+    steps = 8
+    print(f"fused inference n={n} graph={graph}", flush=True)
+    d1, c1, _ = closure.advance(U, grid, params, schemes, bcs, net, 0.4, steps, graph=graph)
+    d2, c2, _ = closure.advance(V, grid, params, schemes, bcs, net, 0.4, steps, graph=graph, layered=True)
+    torch.cuda.synchronize()
+    assert c1.cpu().tolist() == [0, 0, 0] and c2.cpu().tolist() == [0, 0, 0]
+    diff = float((U.buf - V.buf).abs().max())
+    assert diff < 1e-9, diff
+    assert float((d1 - d2).abs().max()) < 1e-12
+
+
 def test_t_end_clips_the_last_step():
     from mfvm import closure
     ns, U, grid, params, schemes, bcs = _kh(32)
