@@ -266,8 +266,8 @@ def ensemble(args, torch):
         # with the other options as chosen
         pers = {}
         for o, t in ens.tuning_log:
-            if all(o.get(k) == best[k] for k in ("split", "order", "remap")) and o["tiles"] == best["tiles"]:
-                pers[f"persist={o['persist']}"] = round(t, 3)
+            if all(o.get(k) == best[k] for k in ("split", "order")) and o["tiles"] == best["tiles"]:
+                pers[f"persist={o['persist']},remap={o['remap']}"] = round(t, 3)
         out[leg] = {"ms": round(ms, 2), "Mcell_updates_per_s": round(cells * steps / ms * 1e-3, 1),
                     "autotune_s": round(tune_s, 2), "trials": len(ens.tuning_log),
                     "autotune_ms_per_step": pers,
