@@ -1045,6 +1045,10 @@ MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl, const S& src)
         // an ulp of hi, changes such a value by at most its own rounding, so
         // the other stages do not read it); the last stage adds its
         // increment to hi + lo in FP64 and splits the sum back into (hi, lo).
+        // The SSP middle stage forms its state as its input (hi + d1, the
+        // window's zp) + (d2 - d1), without reading hi: within about an ulp
+        // of hi + d2, the FP32 state's own precision (the increments, and so
+        // the FP64 accumulation, are unchanged).
         R* lo = static_cast<R*>(a.lo);
         R* dv = static_cast<R*>(a.dinc);
         R* sn = static_cast<R*>(a.snext);
@@ -1054,10 +1058,11 @@ MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl, const S& src)
         const R* U0 = static_cast<const R*>(a.U0);
         double nw[5];
         R hv[5], lv[5], dvv[5], kav[5];
+        const bool needh = fin || a.comp != MFVM_COMP_SSP;
 #pragma unroll
         for (int q = 0; q < 5; ++q) {
           const long long og = oc + q * fs, ii = oi + q * a.il.fs;
-          hv[q] = u0_own ? zp[q] : U0[og];
+          hv[q] = (u0_own || !needh) ? zp[q] : U0[og];
           lv[q] = fin ? lo[og] : R(0);
           dvv[q] = needd ? dv[ii] : R(0);
           kav[q] = needk ? ka[ii] : R(0);
@@ -1068,12 +1073,17 @@ MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl, const S& src)
           const long long og = oc + q * fs, ii = oi + q * a.il.fs;
           const R h = hv[q], l = lv[q];
           if (!fin) {
-            R d;
-            if (a.comp == MFVM_COMP_SET) d = R(a.c0 * dt) * kq;
-            else d = R(a.c0) * (dvv[q] + R(dt) * kq);
+            R d, nxt;
+            if (a.comp == MFVM_COMP_SET) {
+              d = R(a.c0 * dt) * kq;
+              nxt = h + d;
+            } else {
+              d = R(a.c0) * (dvv[q] + R(dt) * kq);
+              nxt = h + (d - dvv[q]);  // h: the stage input (see above)
+            }
             if (own) {
               dv[ii] = d;
-              sn[og] = h + d;
+              sn[og] = nxt;
             }
           } else {
             double inc;
