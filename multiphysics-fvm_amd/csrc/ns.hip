@@ -77,7 +77,11 @@ MFVM_DEV void ns_xyrow_entry(StageArgs& a) {
   spec_args(a, SPEC);
   // consecutive workgroups (neighbouring row chunks of a plane, whose halo
   // rows overlap) on one XCD, sharing its L2
-  const long long task = (long long)xcd_block_1d(a.remap != 0) * XR_WAVES + threadIdx.x / XR_W;
+  // (the wave index through readfirstlane: the compiler then knows the task,
+  // and with it the march's rows and loop exits, to be uniform per wave, and
+  // keeps the loop in scalar branches instead of exec masks)
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / XR_W);
+  const long long task = (long long)xcd_block_1d(a.remap != 0) * XR_WAVES + wave;
   if constexpr (BUF) ns_xyrow_body<LIM, RS, UNIF, R>(a, buf_src<R>(a), task);
   else ns_xyrow_body<LIM, RS, UNIF, R>(a, ptr_src<R>(a), task);
 }

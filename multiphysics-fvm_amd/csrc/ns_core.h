@@ -1329,15 +1329,13 @@ MFVM_DEV void ns_xyrow_strip(const StageArgs& a, const S& src, R* __restrict__ l
   }
   const long long ifs = a.il.fs;
   R* xy = static_cast<R*>(a.xy);
-  // One row of the march; true once row je (the row after the chunk) has
-  // closed the divergence of row je-1.  The loop runs it twice per trip, so
-  // that the carried rows and states swap registers instead of being copied.
-  auto row = [&](int j) -> bool {
-    // ---- x2: states of row j, face j-1/2, the divergence of row j-1 ------------------
+  // x2 part of row j: the states of row j, face j-1/2 and the divergence of
+  // row j-1 (for j > jb).
+  auto x2_row = [&](int j, const R (&c0)[11], const R (&c1)[11]) {
     {
       R w[NQ], wp[NQ], lo2[NQ], hi2[NQ], G[5];
-      q_x2<R>(cw, w);
-      q_x2<R>(cn, wp);  // row j+1 (unused when row j is the ghost row)
+      q_x2<R>(c0, w);
+      q_x2<R>(c1, wp);  // row j+1 (unused when row j is the ghost row)
       x2_states<LIM, UNIF, R>(a, ig, j, m, wm, w, wp, lo2, hi2);
       face_flux<1, RS, R>(a.p, hi2p, lo2, g_area<UNIF, R>(a.geo, 1, ig - 1, j - 1, m - 1), G);
       if (own && j > jb) {
@@ -1353,12 +1351,13 @@ MFVM_DEV void ns_xyrow_strip(const StageArgs& a, const S& src, R* __restrict__ l
         wm[f] = w[f];
       }
     }
-    if (j == je) return true;
-    // ---- x1 faces of row j: states, exchange through LDS, flux differences ---------
+  };
+  // x1 faces of row j: states, exchange through LDS, flux differences.
+  auto x1_row = [&](int j, const R (&c0)[11]) {
     {
       wave_lds_sync();  // the previous row's reads of XF are done
       R q[NQ], lo1[NQ], hi1[NQ];
-      q_x1<R>(cw, q);
+      q_x1<R>(c0, q);
       if (LIM == MFVM_LIM_NONE) {
 #pragma unroll
         for (int f = 0; f < NQ; ++f) lo1[f] = hi1[f] = q[f];
@@ -1399,17 +1398,56 @@ MFVM_DEV void ns_xyrow_strip(const StageArgs& a, const S& src, R* __restrict__ l
 #pragma unroll
       for (int q2 = 0; q2 < 5; ++q2) axp[q2] = F[q2] - src_l[q2 * sl];
     }
-    // ---- next row: j+1 becomes the centre, j+3 loads ---------------------------------
-#pragma unroll
-    for (int f = 0; f < 11; ++f) {
-      cw[f] = cn[f];
-      cn[f] = cnn[f];
-    }
-    load_xy11<R>(src, a.lay.at(ic, min(j + 3, n2 + 1), m), visc, cnn);
-    return false;
   };
-  for (int j = jb;; j += 2) {
-    if (row(j) || row(j + 1)) break;
+  // Rows j (c0) and j+1 (c1) are read, row j+2 (c2) is in flight; row j+3
+  // then loads into c0.
+  auto step = [&](int j, R (&c0)[11], const R (&c1)[11]) {
+    x2_row(j, c0, c1);
+    x1_row(j, c0);
+    load_xy11<R>(src, a.lay.at(ic, min(j + 3, n2 + 1), m), visc, c0);
+  };
+  if constexpr (sizeof(R) == sizeof(double)) {
+    // float64: the rows of the chunk three at a time, the three carried rows
+    // rotating their roles (no copies, and a row loaded here is first read two
+    // rows later); then the last rows one at a time; then the x2 part of row
+    // je (the row after the chunk), which closes the divergence of row je-1.
+    // Every branch here is wave-uniform.
+    int j = jb;
+    for (; j + 3 <= je; j += 3) {
+      step(j, cw, cn);
+      step(j + 1, cn, cnn);
+      step(j + 2, cnn, cw);
+    }
+    for (; j < je; ++j) {
+      step(j, cw, cn);
+#pragma unroll
+      for (int f = 0; f < 11; ++f) {
+        const R t = cw[f];
+        cw[f] = cn[f];
+        cn[f] = cnn[f];
+        cnn[f] = t;
+      }
+    }
+    x2_row(je, cw, cn);
+  } else {
+    // float32 (four waves per SIMD, 128 VGPRs): two rows per trip with the
+    // carried rows copied, which holds fewer of them live than the unrolled
+    // rotation above (that one spills at 128).
+    auto row = [&](int j) -> bool {
+      x2_row(j, cw, cn);
+      if (j == je) return true;
+      x1_row(j, cw);
+#pragma unroll
+      for (int f = 0; f < 11; ++f) {
+        cw[f] = cn[f];
+        cn[f] = cnn[f];
+      }
+      load_xy11<R>(src, a.lay.at(ic, min(j + 3, n2 + 1), m), visc, cnn);
+      return false;
+    };
+    for (int j = jb;; j += 2) {
+      if (row(j) || row(j + 1)) break;
+    }
   }
 }
 
@@ -1419,7 +1457,7 @@ MFVM_DEV void ns_xyrow_strip(const StageArgs& a, const S& src, R* __restrict__ l
 template <int LIM, int RS, bool UNIF, class R, class S>
 MFVM_DEV void ns_xyrow_body(const StageArgs& a, const S& src, long long task) {
   extern __shared__ double smem_raw[];
-  const int wave = threadIdx.x / XR_W;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / XR_W);
   R* lds = reinterpret_cast<R*>(smem_raw) + wave * XR_LDS_ELEMS;
   const int ns = (a.lay.n1 + XR_W - 1) / XR_W;
   const int rows = a.kz;
