@@ -890,36 +890,41 @@ MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl, const S& src)
     if (ZONLY && kb + 1 <= n3 + 1) load12(kb + 1, nx);
   }
 
-  for (int m = kb; m <= ke; ++m) {
-    // ---- x3 face m-1/2 -------------------------------------------------------
-    R fsz[5];
-    R lz_cur[NQ];
-    R nn[12];  // ZONLY: plane m+2, loaded now for the next iteration
-    // EARLY: the reads of the cell finalized in this iteration (m-1) -- U0,
-    // kacc and, in the LATE form, the in-plane divergence -- are issued here,
-    // before the x3 face's work, so that their latency overlaps it (the
-    // float64 x3-only pass stays at 2 waves per SIMD).  The other forms issue
-    // them at the finalize (early reads would cost the compensated x3 pass a
-    // wave per SIMD).
-    R fxy[5], f0[5], f1[5];
-    auto xy_reads = [&](int k) {
-      const long long oi = a.il.at(ic, jc, k);
+  // The march over planes, in two parts per plane m: face(m), the x3 face
+  // m-1/2 and the in-plane work of plane m (it issues the loads of plane m+2
+  // and, EARLY, the reads of cell m-1); close(m), which finalizes cell m-1
+  // (its stores) and rotates the window.
+  R fsz[5];
+  R lz_cur[NQ];
+  R nn[12];  // ZONLY: plane m+2, loaded in face(m)
+  R accm[5], srcm[5];
+  bool src_any = false;
+  // EARLY: the reads of cell m-1, which close(m) finalizes -- U0, kacc and,
+  // in the LATE form, the in-plane divergence -- are issued at the start of
+  // face(m), before the x3 face's work, so that their latency overlaps it (the
+  // float64 x3-only pass stays at 2 waves per SIMD).  The other forms issue
+  // them at the finalize (early reads would cost the compensated x3 pass a
+  // wave per SIMD).
+  R fxy[5], f0[5], f1[5];
+  auto xy_reads = [&](int k) {
+    const long long oi = a.il.at(ic, jc, k);
 #pragma unroll
-      for (int q = 0; q < 5; ++q) fxy[q] = static_cast<const R*>(a.xy)[oi + q * a.il.fs];
-    };
-    // (U0 == U, as in the first stage of a step: U0 of cell m-1 is the
-    // window's zp, read already)
-    const bool u0_own = a.U0 == a.U;
-    auto base_reads = [&](int k) {
-      const long long oc = a.lay.at(ic, jc, k), oi = a.il.at(ic, jc, k);
-      const bool need0 = !u0_own && (a.update == MFVM_UPD_BASE || a.update == MFVM_UPD_SSP || a.update == MFVM_UPD_ACC6);
-      const bool needk = a.update == MFVM_UPD_ACC6 || a.acc == MFVM_ACC_ADD;
+    for (int q = 0; q < 5; ++q) fxy[q] = static_cast<const R*>(a.xy)[oi + q * a.il.fs];
+  };
+  // (U0 == U, as in the first stage of a step: U0 of cell m-1 is the
+  // window's zp, read already)
+  const bool u0_own = a.U0 == a.U;
+  auto base_reads = [&](int k) {
+    const long long oc = a.lay.at(ic, jc, k), oi = a.il.at(ic, jc, k);
+    const bool need0 = !u0_own && (a.update == MFVM_UPD_BASE || a.update == MFVM_UPD_SSP || a.update == MFVM_UPD_ACC6);
+    const bool needk = a.update == MFVM_UPD_ACC6 || a.acc == MFVM_ACC_ADD;
 #pragma unroll
-      for (int q = 0; q < 5; ++q) {
-        f0[q] = need0 ? static_cast<const R*>(a.U0)[oc + q * fs] : R(0);
-        f1[q] = needk ? static_cast<const R*>(a.kacc)[oi + q * a.il.fs] : R(0);
-      }
-    };
+    for (int q = 0; q < 5; ++q) {
+      f0[q] = need0 ? static_cast<const R*>(a.U0)[oc + q * fs] : R(0);
+      f1[q] = needk ? static_cast<const R*>(a.kacc)[oi + q * a.il.fs] : R(0);
+    }
+  };
+  auto face = [&](int m) {
     if constexpr (EARLY) {
       if (m > kb) {
         if constexpr (LATE) xy_reads(m - 1);
@@ -951,8 +956,7 @@ MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl, const S& src)
     // phase 2: F.S of every x1 and x2 face of the tile, each computed once;
     // phase 3: flux differences.  Helper tasks go to the first threads in
     // phase 1 and to the last threads in phase 2, so that no wave takes two.
-    R accm[5], srcm[5];
-    bool src_any = false;
+    src_any = false;
     if (!LATE && m < ke) {
       if constexpr (ZONLY) {
         const long long oi = a.il.at(ic, jc, m);
@@ -964,6 +968,8 @@ MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl, const S& src)
       cell_source<UNIF, R>(a.geo, a.p, ic, jc, m, cu, cu + 5, srcm, src_any);
     }
 
+  };
+  auto close = [&](int m) {
     // ---- finalize cell m-1 (stores from the lanes of interior columns) --------
     if (m > kb) {
       const int k = m - 1;
@@ -1133,6 +1139,24 @@ MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl, const S& src)
 #pragma unroll
         for (int f = 0; f < 12; ++f) nx[f] = nn[f];
       }
+    }
+  };
+  if constexpr (COMP) {
+    // close(m-1) before face(m): no store is issued between a plane's loads
+    // and the end of the trip, where the carried window is handed over, so
+    // that hand-over waits for those loads only, not for the stores to
+    // complete (compensated x3 pass at RT 256^3: 1.57 -> 1.53 ms per step;
+    // the float64 pass measured 1% slower in this order and keeps the other).
+    face(kb);
+    for (int m = kb + 1; m <= ke; ++m) {
+      close(m - 1);
+      face(m);
+    }
+    close(ke);
+  } else {
+    for (int m = kb; m <= ke; ++m) {
+      face(m);
+      close(m);
     }
   }
 
