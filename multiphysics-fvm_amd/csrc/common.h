@@ -44,24 +44,27 @@ template <> MFVM_DEV float qsqrt<float>(float a) { return __builtin_amdgcn_sqrtf
 //
 // The reference multiplies a sign-agreement factor by a magnitude.  When both
 // slopes are nonzero with one sign the factor is exactly +-1, so the product
-// is the magnitude with the sign of b (copysign: bit-identical).  In every
-// other case the reference value is a zero (or NaN when an input is NaN),
-// reproduced by (b + f) * 0 up to the sign of that zero.  MC's 2-to-1 split
-// keeps the reference's +-1/3 factor (slope_limiters.py:66-71).
+// is the magnitude with the sign of b (copysign: bit-identical).  MC's 2-to-1
+// split keeps the reference's +-1/3 factor (slope_limiters.py:66-71).  In
+// every other case the factor is 0, +-1/2 or +-2/3 (or NaN for a NaN input)
+// and the magnitude is 0 unless the factor is 0, so the reference value is
+// +-0 for a finite magnitude and NaN for a NaN or infinite one (0 * inf):
+// the magnitude times 0.  Each form below equals the reference for every
+// input, NaN and infinities included, up to the sign of a zero.
 template <int LIM, class R> MFVM_DEV R limit(R b, R f) {
-  const bool same = ((b > R(0)) & (f > R(0))) | ((b < R(0)) & (f < R(0)));
-  const R zero = (b + f) * R(0);
   if constexpr (LIM == MFVM_LIM_MINMOD) {
     // slope_limiters.py:30-37
+    const bool same = ((b > R(0)) & (f > R(0))) | ((b < R(0)) & (f < R(0)));
     const R least = nmin(fabs(b), fabs(f));
-    return same ? copysign(least, b) : zero;
+    return same ? copysign(least, b) : least * R(0);
   } else if constexpr (LIM == MFVM_LIM_MC) {
     // slope_limiters.py:60-71 (sign sum over three candidates divided by 3)
+    const bool same = ((b > R(0)) & (f > R(0))) | ((b < R(0)) & (f < R(0)));
     const R c = R(0.5) * (b + f);
     const R least = nmin(fabs(c), nmin(fabs(R(2) * b), fabs(R(2) * f)));
     const bool opp = ((b > R(0)) & (f < R(0))) | ((b < R(0)) & (f > R(0)));
     const R third = copysign(R(1.0 / 3.0) * least, c);
-    return same ? copysign(least, b) : (opp ? third : zero);
+    return same ? copysign(least, b) : (opp ? third : least * R(0));
   } else if constexpr (sizeof(R) == sizeof(float)) {
     // float32 van Leer: (2b)|f| * rcp(den) when b f > 0, else 0.  For equal
     // signs this is the reference magnitude (2b)f/den with the sign of b, as
@@ -74,10 +77,17 @@ template <int LIM, class R> MFVM_DEV R limit(R b, R f) {
     const R harm = ((b + b) * fabsf(f)) * __builtin_amdgcn_rcpf(den);
     return (b * f > R(0)) ? harm : R(0);
   } else {
-    // slope_limiters.py:94-103 (harmonic mean on denom > 0, else 0)
+    // slope_limiters.py:94-103: the sign factor times the harmonic mean
+    // num / den (0 where den is 0), num = (2 b) f.  Read off num: num > 0
+    // means one sign and both nonzero, the factor +-1 with the sign of b, so
+    // the value is copysign(num / den, b); otherwise the factor is 0 or +-1/2
+    // (or NaN) times 0 or num / den, which is +-0 for a finite num and NaN for
+    // a NaN or infinite one (0 * -inf where the product overflows): num * 0.
+    // One comparison instead of the four that the sign test takes.
     const R den = fabs(b) + fabs(f);
-    const R harm = qdiv(((R(2) * b) * f), den);
-    return same ? copysign(harm, b) : zero;
+    const R num = (R(2) * b) * f;
+    const R harm = qdiv(num, den);
+    return num > R(0) ? copysign(harm, b) : num * R(0);
   }
 }
 

@@ -111,6 +111,50 @@ def test_rt_pow2_steps_match_oracle(shape, split, monkeypatch):
         assert same(U[k].cpu().numpy(), five[k]), k
 
 
+@pytest.mark.parametrize("split", ["1", "0"], ids=["split", "fused"])
+@pytest.mark.parametrize("lim", ["vanleer", "minmod", "mc"])
+def test_rt_pow2_limiter_edge_slopes_match_oracle(lim, split, monkeypatch):
+    """Slopes at the edges of the limiters (slope_limiters.py) through the
+    power-of-two viscous stage kernels, planted in x1, x2 and x3 rows of an
+    RT 32^3 state: opposite-signed slopes with a subnormal product, one-signed
+    slopes with a subnormal numerator (2 b) f, one-signed slopes whose
+    numerator underflows to zero, and zero slopes beside nonzero ones.
+    compute_rhs equals the oracle's bitwise.
+
+    (States whose flux components overflow are outside the bar: the reference
+    projects every flux on the face normal as n1 F1 + n2 F2 + n3 F3, and
+    0 * inf in a tangential component makes the face flux NaN where these
+    kernels, which form the normal component only, keep it finite.)"""
+    from mfvm.numerics import slope_limiters
+    shape = (32, 32, 32)
+    cfg = bench.rt_config(shape)
+    params = bench.PARAMS
+    fields = [f.copy() for f in bench.rt_fields(shape)]
+    fields[2][10:14, 5, 7] = [0.0, 0.0, 3e-162, 0.0]
+    fields[3][20, 14:17, 9] = [0.0, 3e-162, 0.0]
+    fields[1][3, 25, 20:23] = [0.0, 1e-162, 2e-162]
+    fields[1][7:10, 9, 11] = [0.0, 1e-172, 2e-172]
+    fields[3][15, 3, 27:30] = [-1e-172, -2e-172, -2e-172]
+    U = ons.allocate_u(cfg)
+    for k, f in zip(ons.VARS, fields):
+        U[k][1:-1, 1:-1, 1:-1] = f
+    og.apply_bcs(U, _oracle_bcs())
+    start = {k: v.copy() for k, v in U.items()}
+    limiter = {"vanleer": on.van_leer, "minmod": on.minmod, "mc": on.monotonized_central}[lim]
+    sch = {"riemann_solver": on.hll, "reconstructor": on.muscl, "slope_limiter": limiter,
+           "diffusive_solver": on.centered}
+    with np.errstate(under="ignore"):
+        rhs_o = ons.compute_rhs(U, og.build_grid("cartesian", cfg), params, sch)
+    monkeypatch.setenv("MFVM_SPLIT", split)
+    ns, Ug, grid, gsch, _ = _gpu(shape, start)
+    gsch["slope_limiter"] = {"vanleer": slope_limiters.van_leer, "minmod": slope_limiters.minmod,
+                             "mc": slope_limiters.monotonized_central}[lim]
+    print(f"rt pow2 limiter edges {lim} split={split}: compute_rhs", flush=True)
+    rhs = ns.compute_rhs(Ug, grid, params, gsch)
+    for k in ns.VARS:
+        assert same(rhs[k].cpu().numpy(), rhs_o[k]), k
+
+
 @pytest.mark.parametrize("shape", SHAPES[:1] + SHAPES[2:], ids=["32x32x32", "64x32x16"])
 def test_rt_pow2_compensated_near_oracle(shape):
     """FP32-compensated steps against the oracle's FP64 state: L1 <= 1e-6
