@@ -40,6 +40,54 @@ template <> MFVM_DEV float qdiv<float>(float a, float b) { return a * __builtin_
 template <class R> MFVM_DEV R qsqrt(R a) { return sqrt(a); }
 template <> MFVM_DEV float qsqrt<float>(float a) { return __builtin_amdgcn_sqrtf(a); }
 
+// q[i] = n[i] / d for i < K, each the correctly rounded quotient.
+//
+// float64: the gfx9 division sequence, as the compiler emits it for a / b
+// (v_div_scale of the denominator and of the numerator, v_rcp, four FMAs
+// refining the reciprocal, the quotient and its residual, v_div_fmas,
+// v_div_fixup), with the reciprocal refinement of d formed once for all K
+// quotients instead of once per quotient.  The scaled denominator
+// v_div_scale(d, d, n) depends on n only for a zero n or an extreme exponent
+// of n or n / d; where every lane's K scaled denominators equal the one
+// formed with n = d, each quotient here is the one its own sequence gives.
+// A zero n passes too: v_div_fixup returns +-0 (or NaN for d = 0 or NaN)
+// for it whatever the rest of the sequence computed.  Any other lane sends
+// the wave through K plain divisions.
+template <int K, class R>
+MFVM_DEV void qdiv_by(const R (&n)[K], R d, R (&q)[K]) {
+  if constexpr (sizeof(R) == sizeof(float)) {
+#pragma unroll
+    for (int i = 0; i < K; ++i) q[i] = qdiv(n[i], d);
+  } else {
+    bool fl;
+    const double ds = __builtin_amdgcn_div_scale(d, d, false, &fl);
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+      const double dsi = __builtin_amdgcn_div_scale(n[i], d, false, &fl);
+      ok = ok & ((dsi == ds) | (n[i] == 0.0));
+    }
+    if (__builtin_amdgcn_ballot_w64(!ok) == 0) {
+      const double y0 = __builtin_amdgcn_rcp(ds);
+      const double e0 = __builtin_fma(-ds, y0, 1.0);
+      const double y1 = __builtin_fma(y0, e0, y0);
+      const double e1 = __builtin_fma(-ds, y1, 1.0);
+      const double y2 = __builtin_fma(y1, e1, y1);
+#pragma unroll
+      for (int i = 0; i < K; ++i) {
+        bool sc;
+        const double ns = __builtin_amdgcn_div_scale(n[i], d, true, &sc);
+        const double q0 = ns * y2;
+        const double r = __builtin_fma(-ds, q0, ns);
+        q[i] = __builtin_amdgcn_div_fixup(__builtin_amdgcn_div_fmas(r, y2, q0, sc), d, n[i]);
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < K; ++i) q[i] = n[i] / d;
+    }
+  }
+}
+
 // Limited slope, reference slope_limiters.py.  LIM: 1 minmod, 2 MC, 3 van Leer.
 //
 // The reference multiplies a sign-agreement factor by a magnitude.  When both

@@ -441,10 +441,19 @@ MFVM_DEV void cell_states(const StageArgs& a, const S& src, int i, int j, int k,
 template <int D, int RS, class R>
 MFVM_DEV void face_flux(const NSPar& p, const R l[NQ], const R r[NQ], R area, R fsout[5]) {
   const R rl = l[0], rr = r[0];
-  const R vl[3] = {qdiv(l[1], rl), qdiv(l[2], rl), qdiv(l[3], rl)};
-  const R vr[3] = {qdiv(r[1], rr), qdiv(r[2], rr), qdiv(r[3], rr)};
-  const R El = qdiv(l[4], rl), Er = qdiv(r[4], rr);
   const R Pl = l[5], Pr = r[5];
+  // The five quotients over each side's density: velocity, specific energy
+  // and gamma P / rho for the sound speed (:309).
+  R ql[5], qr[5];
+  {
+    const R nl[5] = {l[1], l[2], l[3], l[4], R(p.gamma) * Pl};
+    const R nr[5] = {r[1], r[2], r[3], r[4], R(p.gamma) * Pr};
+    qdiv_by<5, R>(nl, rl, ql);
+    qdiv_by<5, R>(nr, rr, qr);
+  }
+  const R vl[3] = {ql[0], ql[1], ql[2]};
+  const R vr[3] = {qr[0], qr[1], qr[2]};
+  const R El = ql[3], Er = qr[3];
   R fl[5], fr[5];
   fl[0] = rl * vl[D];
   fr[0] = rr * vr[D];
@@ -460,8 +469,8 @@ MFVM_DEV void face_flux(const NSPar& p, const R l[NQ], const R r[NQ], R area, R 
   fl[4] = ((rl * El) + Pl) * vl[D];
   fr[4] = ((rr * Er) + Pr) * vr[D];
   // :309  cs = sqrt(ga*P/rho)
-  const R csl = qsqrt(qdiv((R(p.gamma) * Pl), rl));
-  const R csr = qsqrt(qdiv((R(p.gamma) * Pr), rr));
+  const R csl = qsqrt(ql[4]);
+  const R csr = qsqrt(qr[4]);
   R F[5];
   if constexpr (RS == MFVM_RS_HLL) {
     // Wave set {vn, vn-cs, vn+cs}: its min is vn-cs and its max vn+cs.
@@ -483,8 +492,10 @@ MFVM_DEV void face_flux(const NSPar& p, const R l[NQ], const R r[NQ], R area, R 
     } else {
       const R den = smax - smin;
       const R sm = smin * smax;
+      R nm[5];
 #pragma unroll
-      for (int q = 0; q < 5; ++q) F[q] = qdiv(((smax * fl[q] - smin * fr[q]) + sm * (r[q] - l[q])), den);
+      for (int q = 0; q < 5; ++q) nm[q] = (smax * fl[q] - smin * fr[q]) + sm * (r[q] - l[q]);
+      qdiv_by<5, R>(nm, den, F);
     }
   } else {
     // max |{vn, vn-cs, vn+cs}| is |vn| + cs, rounded once.
