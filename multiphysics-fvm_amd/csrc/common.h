@@ -54,6 +54,21 @@ template <> MFVM_DEV float qsqrt<float>(float a) { return __builtin_amdgcn_sqrtf
 // for it whatever the rest of the sequence computed.  Any other lane sends
 // the wave through K plain divisions.
 template <int K, class R>
+MFVM_DEV void qdiv_by(const R (&n)[K], R d, R (&q)[K]);
+
+// The same quotients, correctly rounded in float32 as well (IEEE float
+// division there, where qdiv_by takes the hardware reciprocal).
+template <int K, class R>
+MFVM_DEV void ieee_div_by(const R (&n)[K], R d, R (&q)[K]) {
+  if constexpr (sizeof(R) == sizeof(float)) {
+#pragma unroll
+    for (int i = 0; i < K; ++i) q[i] = n[i] / d;
+  } else {
+    qdiv_by<K, R>(n, d, q);
+  }
+}
+
+template <int K, class R>
 MFVM_DEV void qdiv_by(const R (&n)[K], R d, R (&q)[K]) {
   if constexpr (sizeof(R) == sizeof(float)) {
 #pragma unroll

@@ -104,10 +104,12 @@ __global__ __launch_bounds__(256) void ns_z_kernel(StageArgs a) {
   ns_stage_body<LIM, RS, UNIF, double, false, true>(a, grid_tile(a), ptr_src<double>(a));
 }
 
-// FIN: the final stage (float64 accumulation and CFL partials) at two waves
-// per SIMD; the other stages leave that code out and fit three (144 VGPRs).
+// FIN: the final stage (float64 accumulation and CFL partials), held to four
+// waves per SIMD (128 VGPRs, 2 spilled): 0.73 -> 0.62 ms at RT 256^3 against
+// the three its 133 VGPRs give unbounded.  The other stages leave that code
+// out and fit four (114 VGPRs).
 template <int LIM, int RS, bool UNIF, bool FIN, int SPEC>
-__global__ __launch_bounds__(256, FIN ? 2 : 3) void ns_z_comp_kernel(StageArgs a) {
+__global__ __launch_bounds__(256, FIN ? 4 : 3) void ns_z_comp_kernel(StageArgs a) {
   spec_args(a, SPEC);
   ns_stage_body<LIM, RS, UNIF, float, true, true, PtrSrc<float>, FIN ? 1 : 0>(a, grid_tile(a), ptr_src<float>(a));
 }

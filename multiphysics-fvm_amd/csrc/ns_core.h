@@ -526,7 +526,10 @@ MFVM_DEV void cell_source(const Geo& g, const NSPar& p, int i, int j, int k, con
   if ((UNIF || !g.curv) && !p.grav) return;
   any = true;
   const R rho = u[0];
-  const R v1 = u[1] / rho, v2 = u[2] / rho, v3 = u[3] / rho;
+  const R n3[3] = {u[1], u[2], u[3]};
+  R q3[3];
+  ieee_div_by<3, R>(n3, rho, q3);
+  const R v1 = q3[0], v2 = q3[1], v3 = q3[2];
   R sg1 = R(0), sg2 = R(0), sg3 = R(0);
   if (!UNIF && g.curv) {
     const int a = i - 1, b = j - 1, e = k - 1;
@@ -573,8 +576,12 @@ template <bool UNIF, class R>
 MFVM_DEV void cell_cfl(const Geo& g, const NSPar& p, int i, int j, int k, const R u[5], R& rate,
                        R& nu) {
   const R rho = u[0];
-  const R v1 = u[1] / rho, v2 = u[2] / rho, v3 = u[3] / rho;
-  const R E = u[4] / rho;
+  // v, E and mu / rho: five quotients over rho (IEEE, shared reciprocal)
+  const R n5[5] = {u[1], u[2], u[3], u[4], R(p.mu)};
+  R q5[5];
+  ieee_div_by<5, R>(n5, rho, q5);
+  const R v1 = q5[0], v2 = q5[1], v3 = q5[2];
+  const R E = q5[3];
   const R P = (R(p.gm1) * rho) * (E - (((v1 * v1) + (v2 * v2)) + (v3 * v3)) / R(2));
   const R cs = sqrt((R(p.gamma) * P) / rho);
   const R a1 = fabs(v1) + cs, a2 = fabs(v2) + cs, a3 = fabs(v3) + cs;
@@ -582,7 +589,7 @@ MFVM_DEV void cell_cfl(const Geo& g, const NSPar& p, int i, int j, int k, const 
   const R r2 = div_len<R>(g, 1, a2, cell_len<1, UNIF, R>(g, i, j, k));
   const R r3 = div_len<R>(g, 2, a3, cell_len<2, UNIF, R>(g, i, j, k));
   rate = nmax(nmax(r1, r2), r3);
-  nu = R(p.mu) / rho;
+  nu = q5[4];
 }
 
 // Block max of (rate, nu) -> partials[2*block], NaN-propagating.
