@@ -449,8 +449,11 @@ struct NSCtx {
 };
 static int ns_pre(void* c, const void* in, void* C, void* stream) {
   NSCtx* x = static_cast<NSCtx*>(c);
-  if (x->p->dtype == MFVM_F32)  // the float32 stage forms P from U: the closure leaves it out
+  // the float32 stage and the float64 split stage form P from U: the closure leaves it out
+  if (x->p->dtype == MFVM_F32)
     return hip_check(launch_closure<float>(x->g, x->p, in, C, static_cast<hipStream_t>(stream), false), "ns_closure");
+  if (x->xy)
+    return hip_check(launch_closure<double>(x->g, x->p, in, C, static_cast<hipStream_t>(stream), false), "ns_closure");
   return mfvm_ns_closure(x->g, x->p, in, C, stream);
 }
 static int ns_stage_op(void* c, const void* in, const void* C, const StageIO& io, double* partials, void* stream) {

@@ -82,8 +82,9 @@ MFVM_DEV void ns_xyrow_entry(StageArgs& a) {
   // keeps the loop in scalar branches instead of exec masks)
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / XR_W);
   const long long task = (long long)xcd_block_1d(a.remap != 0) * XR_WAVES + wave;
-  if constexpr (BUF) ns_xyrow_body<LIM, RS, UNIF, R>(a, buf_src<R>(a), task);
-  else ns_xyrow_body<LIM, RS, UNIF, R>(a, ptr_src<R>(a), task);
+  // (P formed from U: the closure before a split stage leaves it out)
+  if constexpr (BUF) ns_xyrow_body<LIM, RS, UNIF, R>(a, buf_src<R, true>(a), task);
+  else ns_xyrow_body<LIM, RS, UNIF, R>(a, ptr_src<R, true>(a), task);
 }
 template <int LIM, int RS, bool UNIF, class R, bool BUF, int SPEC>
 __global__ __launch_bounds__(XR_W * XR_WAVES) void ns_xyrow_kernel(StageArgs a) {
@@ -101,7 +102,7 @@ void ns_xyrow_f32_kernel(StageArgs a) {
 template <int LIM, int RS, bool UNIF, int SPEC>
 __global__ __launch_bounds__(256) void ns_z_kernel(StageArgs a) {
   spec_args(a, SPEC);
-  ns_stage_body<LIM, RS, UNIF, double, false, true>(a, grid_tile(a), ptr_src<double>(a));
+  ns_stage_body<LIM, RS, UNIF, double, false, true>(a, grid_tile(a), ptr_src<double, true>(a));
 }
 
 // FIN: the final stage (float64 accumulation and CFL partials), held to four
@@ -161,12 +162,11 @@ static int closure_kz(bool f32) {
   return env > 0 ? env : (f32 ? CLOSURE_KZ_F32 : CLOSURE_KZ_F64);
 }
 
-// with_p = false (float32 only): P is left out -- the float32 stage passes
-// form it from U -- and an inviscid closure launches nothing.
+// with_p = false: P is left out -- the float32 stage passes and the float64
+// split passes form it from U -- and an inviscid closure launches nothing.
 template <class R>
 hipError_t launch_closure(const mfvm_grid* g, const mfvm_ns_params* p, const void* U, void* C,
                           hipStream_t s, bool with_p) {
-  if (sizeof(R) == sizeof(double)) with_p = true;
   if (!with_p && p->mu == 0.0) return hipSuccess;
   const Layout lay = make_layout(g->n[0], g->n[1], g->n[2]);
   const Geo geo = make_geo(g);
@@ -199,7 +199,7 @@ hipError_t launch_closure(const mfvm_grid* g, const mfvm_ns_params* p, const voi
     if (with_p) {
       MFVM_CLOSURE_VISC(true)
     } else {
-      MFVM_CLOSURE_VISC(sizeof(R) == sizeof(double))
+      MFVM_CLOSURE_VISC(false)
     }
 #undef MFVM_CLOSURE_VISC
   }

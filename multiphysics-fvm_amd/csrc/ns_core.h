@@ -82,9 +82,12 @@ struct StageArgs {
 // ---------------------------------------------------------------------------
 // :100  P = (ga-1)*rho*(E - (v1**2+v2**2+v3**2)/2), from the cell's U.
 template <class R>
-MFVM_DEV R closure_p(const NSPar& p, R rho, R v1, R v2, R v3, R rhoE) {
-  const R E = qdiv(rhoE, rho);
+MFVM_DEV R closure_p_e(const NSPar& p, R rho, R v1, R v2, R v3, R E) {
   return (R(p.gm1) * rho) * (E - (((v1 * v1) + (v2 * v2)) + (v3 * v3)) / R(2));
+}
+template <class R>
+MFVM_DEV R closure_p(const NSPar& p, R rho, R v1, R v2, R v3, R rhoE) {
+  return closure_p_e(p, rho, v1, v2, v3, qdiv(rhoE, rho));
 }
 
 // :94-97 primitive velocity of the cell at offset o.
@@ -291,16 +294,21 @@ MFVM_DEV void closure_march_body(const Layout& lay, const Geo& geo, const NSPar&
 // conserved fields, c(o, f) of the closure (P and the Stokes tensor), both
 // field-major with a field stride of fs elements.
 //
-// In float32 the pressure is not read from the closure but formed from the
-// cell's U, as the closure forms it (navier_stokes.py:94-100; p_from_u): a
-// few FP32 operations instead of 4 B of memory traffic per read.
+// PU: the pressure is not read from the closure but formed from the cell's U,
+// as the closure forms it (navier_stokes.py:94-100; p_from_u): four quotients
+// over rho and a few products instead of a read (8 B per cell in float64).
+// Bit-identical to the closure's P, so the float64 passes may take either.
+// The float32 stage passes and the float64 split passes take it; the closure
+// before them then leaves P out.
 template <class R, class S>
 MFVM_DEV R p_from_u(const S& src, long long o) {
   const R rho = src.u(o, 0);
-  return closure_p(src.par, rho, qdiv(src.u(o, 1), rho), qdiv(src.u(o, 2), rho), qdiv(src.u(o, 3), rho),
-                   src.u(o, 4));
+  const R n4[4] = {src.u(o, 1), src.u(o, 2), src.u(o, 3), src.u(o, 4)};
+  R q4[4];
+  qdiv_by<4, R>(n4, rho, q4);
+  return closure_p_e(src.par, rho, q4[0], q4[1], q4[2], q4[3]);
 }
-template <class R>
+template <class R, bool PU = sizeof(R) == sizeof(float)>
 struct PtrSrc {
   const R* __restrict__ U;
   const R* __restrict__ C;
@@ -308,7 +316,7 @@ struct PtrSrc {
   NSPar par;
   MFVM_DEV R u(long long o, int f) const { return U[o + f * fs]; }
   MFVM_DEV R c(long long o, int f) const {
-    if (sizeof(R) == sizeof(float) && f == C_P) return p_from_u<R>(*this, o);
+    if (PU && f == C_P) return p_from_u<R>(*this, o);
     return C[o + f * fs];
   }
 };
@@ -318,7 +326,7 @@ struct PtrSrc {
 // cell need no per-field address arithmetic and no 64-bit base per field
 // (fewer scalar registers, fewer of them spilled).  Valid while every array
 // is below 4 GiB (buf_src_ok).
-template <class R>
+template <class R, bool PU = sizeof(R) == sizeof(float)>
 struct BufSrc {
   __amdgpu_buffer_rsrc_t U, C;
   unsigned fsb;  // field stride in bytes
@@ -333,7 +341,7 @@ struct BufSrc {
   }
   MFVM_DEV R u(long long o, int f) const { return ld(U, o, f); }
   MFVM_DEV R c(long long o, int f) const {
-    if (sizeof(R) == sizeof(float) && f == C_P) return p_from_u<R>(*this, o);
+    if (PU && f == C_P) return p_from_u<R>(*this, o);
     return ld(C, o, f);
   }
 };
@@ -347,9 +355,9 @@ MFVM_DEV void spec_args(StageArgs& a, int spec) {
   if (spec == 2) a.p.visc = 1;
 }
 
-template <class R>
-MFVM_DEV PtrSrc<R> ptr_src(const StageArgs& a) {
-  return PtrSrc<R>{static_cast<const R*>(a.U), static_cast<const R*>(a.C), a.lay.fs, a.p};
+template <class R, bool PU = sizeof(R) == sizeof(float)>
+MFVM_DEV PtrSrc<R, PU> ptr_src(const StageArgs& a) {
+  return PtrSrc<R, PU>{static_cast<const R*>(a.U), static_cast<const R*>(a.C), a.lay.fs, a.p};
 }
 
 // Buffer resource over [p, p + 4 GiB): raw (stride 0), the gfx9 data-format
@@ -358,9 +366,9 @@ MFVM_DEV __amdgpu_buffer_rsrc_t buf_rsrc(const void* p) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, -1, 0x00020000);
 }
 
-template <class R>
-MFVM_DEV BufSrc<R> buf_src(const StageArgs& a) {
-  return BufSrc<R>{buf_rsrc(a.U), buf_rsrc(a.C), (unsigned)(a.lay.fs * (long long)sizeof(R)), a.p};
+template <class R, bool PU = sizeof(R) == sizeof(float)>
+MFVM_DEV BufSrc<R, PU> buf_src(const StageArgs& a) {
+  return BufSrc<R, PU>{buf_rsrc(a.U), buf_rsrc(a.C), (unsigned)(a.lay.fs * (long long)sizeof(R)), a.p};
 }
 
 template <int D, class R, class S>
@@ -1244,10 +1252,12 @@ constexpr int XR_WAVES = 4;   // independent waves per workgroup
 // and x1 fields EQ[XR_ROWS][2][NQ] of cells i0-1 and i0+64.
 constexpr int XR_LDS_ELEMS = NQ * XR_W + XR_ROWS * NQ + XR_ROWS * 5 + XR_ROWS * 2 * NQ;
 
-// This wave's LDS writes are visible to its other lanes' later reads.
+// This wave's LDS writes are visible to its other lanes' later reads: the
+// compiler keeps the LDS accesses on either side in program order, and the
+// LDS performs one wave's accesses in issue order, so no wait for the writes
+// to complete is needed (an lgkmcnt(0) wait here measured 1% slower).
 MFVM_DEV void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
