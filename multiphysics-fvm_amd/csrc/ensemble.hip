@@ -143,8 +143,10 @@ __global__ __launch_bounds__(kEnsBlock, 2) void ens_stage_comp_kernel(const EnsM
 
 // Split stage over the work list: the x1/x2 pass of every member's tiles,
 // then the x3 pass (as ns_xy_kernel / ns_z_kernel of the single-grid path).
+// The x1/x2 pass is held to four (float32) or three (float64) waves per SIMD
+// (unbounded, the float64 pass took 174 VGPRs and two waves).
 template <int LIM, int RS, bool UNIF, class R, int SPEC>
-__global__ __launch_bounds__(kEnsBlock) void ens_xy_kernel(const EnsMem* __restrict__ mem,
+__global__ __launch_bounds__(kEnsBlock, sizeof(R) == sizeof(float) ? 4 : 3) void ens_xy_kernel(const EnsMem* __restrict__ mem,
                                                            const StageArgs* __restrict__ st,
                                                            const EnsWork* __restrict__ work, long long nwork,
                                                            int remap) {

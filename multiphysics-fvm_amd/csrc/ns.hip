@@ -112,7 +112,7 @@ __global__ __launch_bounds__(256) void ns_z_kernel(StageArgs a) {
 template <int LIM, int RS, bool UNIF, bool FIN, int SPEC>
 __global__ __launch_bounds__(256, FIN ? 4 : 3) void ns_z_comp_kernel(StageArgs a) {
   spec_args(a, SPEC);
-  ns_stage_body<LIM, RS, UNIF, float, true, true, PtrSrc<float>, FIN ? 1 : 0>(a, grid_tile(a), ptr_src<float>(a));
+  ns_stage_body<LIM, RS, UNIF, float, true, true, PtrSrc<float>, FIN ? 1 : 0, true>(a, grid_tile(a), ptr_src<float>(a));
 }
 
 // Standalone CFL partials (navier_stokes.py:512-583), same grid as the stage.

@@ -839,7 +839,7 @@ MFVM_DEV void inplane_div(const StageArgs& a, const S& src, R* sm, int BX, int B
 // FIN (compensated stages): 2 -- whether this is the final stage is read
 // from a.comp; 0 / 1 -- known at compile time (a.comp must agree), which
 // leaves the float64 accumulation and CFL code out of the other stages.
-template <int LIM, int RS, bool UNIF, class R, bool COMP, bool ZONLY, class S, int FIN = 2>
+template <int LIM, int RS, bool UNIF, class R, bool COMP, bool ZONLY, class S, int FIN = 2, bool FC = false>
 MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl, const S& src) {
   extern __shared__ double smem_raw[];
   R* sm = reinterpret_cast<R*>(smem_raw);
@@ -916,274 +916,532 @@ MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl, const S& src)
     if (ZONLY && kb + 1 <= n3 + 1) load12(kb + 1, nx);
   }
 
-  // The march over planes, in two parts per plane m: face(m), the x3 face
-  // m-1/2 and the in-plane work of plane m (it issues the loads of plane m+2
-  // and, EARLY, the reads of cell m-1); close(m), which finalizes cell m-1
-  // (its stores) and rotates the window.
-  R fsz[5];
-  R lz_cur[NQ];
-  R nn[12];  // ZONLY: plane m+2, loaded in face(m)
-  R accm[5], srcm[5];
-  bool src_any = false;
-  // EARLY: the reads of cell m-1, which close(m) finalizes -- U0, kacc and,
-  // in the LATE form, the in-plane divergence -- are issued at the start of
-  // face(m), before the x3 face's work, so that their latency overlaps it (the
-  // float64 x3-only pass stays at 2 waves per SIMD).  The other forms issue
-  // them at the finalize (early reads would cost the compensated x3 pass a
-  // wave per SIMD).
-  R fxy[5], f0[5], f1[5];
-  auto xy_reads = [&](int k) {
-    const long long oi = a.il.at(ic, jc, k);
-#pragma unroll
-    for (int q = 0; q < 5; ++q) fxy[q] = static_cast<const R*>(a.xy)[oi + q * a.il.fs];
-  };
-  // (U0 == U, as in the first stage of a step: U0 of cell m-1 is the
-  // window's zp, read already)
-  const bool u0_own = a.U0 == a.U;
-  auto base_reads = [&](int k) {
-    const long long oc = a.lay.at(ic, jc, k), oi = a.il.at(ic, jc, k);
-    const bool need0 = !u0_own && (a.update == MFVM_UPD_BASE || a.update == MFVM_UPD_SSP || a.update == MFVM_UPD_ACC6);
-    const bool needk = a.update == MFVM_UPD_ACC6 || a.acc == MFVM_ACC_ADD;
-#pragma unroll
-    for (int q = 0; q < 5; ++q) {
-      f0[q] = need0 ? static_cast<const R*>(a.U0)[oc + q * fs] : R(0);
-      f1[q] = needk ? static_cast<const R*>(a.kacc)[oi + q * a.il.fs] : R(0);
-    }
-  };
-  auto face = [&](int m) {
-    if constexpr (EARLY) {
-      if (m > kb) {
-        if constexpr (LATE) xy_reads(m - 1);
-        base_reads(m - 1);
-      }
-    }
-    {
-      R zc[NQ], lo[NQ];
-      zs(cu, zc);
-      if constexpr (ZONLY) {
-        if (m + 2 <= n3 + 1 && m + 1 <= ke) load12(m + 2, nn);
-      } else {
-        if (m + 1 <= n3 + 1) load12(m + 1, nx);
-      }
-      if (m == n3 + 1 || LIM == MFVM_LIM_NONE) {
-#pragma unroll
-        for (int f = 0; f < NQ; ++f) lo[f] = lz_cur[f] = zc[f];
-      } else {
-        R zn[NQ];
-        zs(nx, zn);
-        slope_states<2, LIM, UNIF, R>(a.geo, ic, jc, m, visc ? 9 : 6, zp, zc, zn, lo, lz_cur);
-      }
-      const R s3 = g_area<UNIF, R>(a.geo, 2, ic - 1, jc - 1, m - 1);
-      face_flux<2, RS, R>(a.p, lz_prev, lo, s3, fsz);
-    }
-
-    // ---- in-plane faces of plane m (all threads take part in the barriers) ----
-    // Phase 1: x1 and x2 states of the own cells and of the tile-edge cells;
-    // phase 2: F.S of every x1 and x2 face of the tile, each computed once;
-    // phase 3: flux differences.  Helper tasks go to the first threads in
-    // phase 1 and to the last threads in phase 2, so that no wave takes two.
-    src_any = false;
-    if (!LATE && m < ke) {
-      if constexpr (ZONLY) {
-        const long long oi = a.il.at(ic, jc, m);
-#pragma unroll
-        for (int q = 0; q < 5; ++q) accm[q] = static_cast<const R*>(a.xy)[oi + q * a.il.fs];
-      } else {
-        inplane_div<LIM, RS, UNIF, R>(a, src, sm, BX, BY, tx, ty, tid, i0, j0, i, j, own, m, cu, accm);
-      }
-      cell_source<UNIF, R>(a.geo, a.p, ic, jc, m, cu, cu + 5, srcm, src_any);
-    }
-
-  };
-  auto close = [&](int m) {
-    // ---- finalize cell m-1 (stores from the lanes of interior columns) --------
-    if (m > kb) {
-      const int k = m - 1;
-      R kr[5];
-      R vol;
-      const bool vp2 = a.geo.vpow2 != 0;
-      if (!vp2) vol = g_vol<UNIF, R>(a.geo, ic, jc, k);
-      if constexpr (LATE) {
-        if constexpr (EARLY) {
-#pragma unroll
-          for (int q = 0; q < 5; ++q) acc_prev[q] = fxy[q];
-        } else {
-          const long long oi = a.il.at(ic, jc, k);
-#pragma unroll
-          for (int q = 0; q < 5; ++q) acc_prev[q] = static_cast<const R*>(a.xy)[oi + q * a.il.fs];
-        }
-        // (uniform Cartesian: the source reads U only, never the closure)
-        cell_source<UNIF, R>(a.geo, a.p, ic, jc, k, zp, zp + 5, src_prev, src_any_prev);
-      }
-#pragma unroll
-      for (int q = 0; q < 5; ++q) {
-        const R x = -((acc_prev[q] + fsz[q]) - fsz_prev[q]);
-        kr[q] = vp2 ? x * R(a.geo.vinv) : qdiv(x, vol);
-        if (src_any_prev) kr[q] = kr[q] + src_prev[q];
-      }
-      const long long oc = a.lay.at(ic, jc, k);
+  // Two forms of the march over planes, equal in results.  FC (the
+  // compensated x3-only pass): two parts per plane, closing the previous
+  // plane before the next plane's loads; it also leaves the final stage few
+  // enough registers for four waves per SIMD (with both, 1.57 -> 1.42 ms of
+  // x3 passes per RT 256^3 step).  Otherwise one loop body per plane: the
+  // face/close form instantiated in a file also changed the compiler's
+  // allocation of that file's other kernels (the ensemble's x1/x2 pass,
+  // 102 -> 150 VGPRs).
+  if constexpr (FC) {
+    // The march over planes, in two parts per plane m: face(m), the x3 face
+    // m-1/2 and the in-plane work of plane m (it issues the loads of plane m+2
+    // and, EARLY, the reads of cell m-1); close(m), which finalizes cell m-1
+    // (its stores) and rotates the window.
+    R fsz[5];
+    R lz_cur[NQ];
+    R nn[12];  // ZONLY: plane m+2, loaded in face(m)
+    R accm[5], srcm[5];
+    bool src_any = false;
+    // EARLY: the reads of cell m-1, which close(m) finalizes -- U0, kacc and,
+    // in the LATE form, the in-plane divergence -- are issued at the start of
+    // face(m), before the x3 face's work, so that their latency overlaps it (the
+    // float64 x3-only pass stays at 2 waves per SIMD).  The other forms issue
+    // them at the finalize (early reads would cost the compensated x3 pass a
+    // wave per SIMD).
+    R fxy[5], f0[5], f1[5];
+    auto xy_reads = [&](int k) {
       const long long oi = a.il.at(ic, jc, k);
-      R* out = static_cast<R*>(a.out);
-      R* ka = static_cast<R*>(a.kacc);
-      // The cell's reads of U0, kacc, lo and dinc are all issued before its
-      // first store (the arrays do not alias the outputs), so that they take
-      // one memory round trip instead of one per field.
-      if constexpr (!COMP) {
-        R nu_[5], u0v[5], kav[5];
-        const R dtr = R(dt);
-        if constexpr (EARLY) {
-#pragma unroll
-          for (int q = 0; q < 5; ++q) {
-            u0v[q] = u0_own ? zp[q] : f0[q];
-            kav[q] = f1[q];
-          }
+  #pragma unroll
+      for (int q = 0; q < 5; ++q) fxy[q] = static_cast<const R*>(a.xy)[oi + q * a.il.fs];
+    };
+    // (U0 == U, as in the first stage of a step: U0 of cell m-1 is the
+    // window's zp, read already)
+    const bool u0_own = a.U0 == a.U;
+    auto base_reads = [&](int k) {
+      const long long oc = a.lay.at(ic, jc, k), oi = a.il.at(ic, jc, k);
+      const bool need0 = !u0_own && (a.update == MFVM_UPD_BASE || a.update == MFVM_UPD_SSP || a.update == MFVM_UPD_ACC6);
+      const bool needk = a.update == MFVM_UPD_ACC6 || a.acc == MFVM_ACC_ADD;
+  #pragma unroll
+      for (int q = 0; q < 5; ++q) {
+        f0[q] = need0 ? static_cast<const R*>(a.U0)[oc + q * fs] : R(0);
+        f1[q] = needk ? static_cast<const R*>(a.kacc)[oi + q * a.il.fs] : R(0);
+      }
+    };
+    auto face = [&](int m) {
+      if constexpr (EARLY) {
+        if (m > kb) {
+          if constexpr (LATE) xy_reads(m - 1);
+          base_reads(m - 1);
+        }
+      }
+      {
+        R zc[NQ], lo[NQ];
+        zs(cu, zc);
+        if constexpr (ZONLY) {
+          if (m + 2 <= n3 + 1 && m + 1 <= ke) load12(m + 2, nn);
         } else {
-          const R* U0 = static_cast<const R*>(a.U0);
-          const bool need0 = a.update == MFVM_UPD_BASE || a.update == MFVM_UPD_SSP || a.update == MFVM_UPD_ACC6;
-          const bool needk = a.update == MFVM_UPD_ACC6 || a.acc == MFVM_ACC_ADD;
-#pragma unroll
-          for (int q = 0; q < 5; ++q) {
-            u0v[q] = u0_own ? zp[q] : (need0 ? U0[oc + q * fs] : R(0));
-            kav[q] = needk ? ka[oi + q * a.il.fs] : R(0);
+          if (m + 1 <= n3 + 1) load12(m + 1, nx);
+        }
+        if (m == n3 + 1 || LIM == MFVM_LIM_NONE) {
+  #pragma unroll
+          for (int f = 0; f < NQ; ++f) lo[f] = lz_cur[f] = zc[f];
+        } else {
+          R zn[NQ];
+          zs(nx, zn);
+          slope_states<2, LIM, UNIF, R>(a.geo, ic, jc, m, visc ? 9 : 6, zp, zc, zn, lo, lz_cur);
+        }
+        const R s3 = g_area<UNIF, R>(a.geo, 2, ic - 1, jc - 1, m - 1);
+        face_flux<2, RS, R>(a.p, lz_prev, lo, s3, fsz);
+      }
+
+      // ---- in-plane faces of plane m (all threads take part in the barriers) ----
+      // Phase 1: x1 and x2 states of the own cells and of the tile-edge cells;
+      // phase 2: F.S of every x1 and x2 face of the tile, each computed once;
+      // phase 3: flux differences.  Helper tasks go to the first threads in
+      // phase 1 and to the last threads in phase 2, so that no wave takes two.
+      src_any = false;
+      if (!LATE && m < ke) {
+        if constexpr (ZONLY) {
+          const long long oi = a.il.at(ic, jc, m);
+  #pragma unroll
+          for (int q = 0; q < 5; ++q) accm[q] = static_cast<const R*>(a.xy)[oi + q * a.il.fs];
+        } else {
+          inplane_div<LIM, RS, UNIF, R>(a, src, sm, BX, BY, tx, ty, tid, i0, j0, i, j, own, m, cu, accm);
+        }
+        cell_source<UNIF, R>(a.geo, a.p, ic, jc, m, cu, cu + 5, srcm, src_any);
+      }
+
+    };
+    auto close = [&](int m) {
+      // ---- finalize cell m-1 (stores from the lanes of interior columns) --------
+      if (m > kb) {
+        const int k = m - 1;
+        R kr[5];
+        R vol;
+        const bool vp2 = a.geo.vpow2 != 0;
+        if (!vp2) vol = g_vol<UNIF, R>(a.geo, ic, jc, k);
+        if constexpr (LATE) {
+          if constexpr (EARLY) {
+  #pragma unroll
+            for (int q = 0; q < 5; ++q) acc_prev[q] = fxy[q];
+          } else {
+            const long long oi = a.il.at(ic, jc, k);
+  #pragma unroll
+            for (int q = 0; q < 5; ++q) acc_prev[q] = static_cast<const R*>(a.xy)[oi + q * a.il.fs];
           }
+          // (uniform Cartesian: the source reads U only, never the closure)
+          cell_source<UNIF, R>(a.geo, a.p, ic, jc, k, zp, zp + 5, src_prev, src_any_prev);
         }
-#pragma unroll
+  #pragma unroll
         for (int q = 0; q < 5; ++q) {
-          const R uq = zp[q];  // stage input U at cell m-1
-          R o = uq;
-          switch (a.update) {
-            case MFVM_UPD_EULER: o = uq + dtr * kr[q]; break;
-            case MFVM_UPD_BASE: o = u0v[q] + R(a.c0 * dt) * kr[q]; break;
-            case MFVM_UPD_SSP: o = (R(a.c0) * u0v[q]) + (R(a.c1) * (uq + dtr * kr[q])); break;
-            case MFVM_UPD_ACC6: o = u0v[q] + R(dt / 6.0) * (kav[q] + kr[q]); break;
-            default: break;
-          }
-          nu_[q] = o;
-          if (own) {
-            if (a.update != MFVM_UPD_NONE) out[oc + q * fs] = o;
-            if (a.acc == MFVM_ACC_SET) ka[oi + q * a.il.fs] = kr[q];
-            else if (a.acc == MFVM_ACC_ADD) ka[oi + q * a.il.fs] = kav[q] + R(a.acc_coef) * kr[q];
-          }
+          const R x = -((acc_prev[q] + fsz[q]) - fsz_prev[q]);
+          kr[q] = vp2 ? x * R(a.geo.vinv) : qdiv(x, vol);
+          if (src_any_prev) kr[q] = kr[q] + src_prev[q];
         }
-        if (a.partials) {
-          R rate, nu;
-          cell_cfl<UNIF, R>(a.geo, a.p, ic, jc, k, nu_, rate, nu);
-          rate_max = own ? nmax(rate_max, double(rate)) : rate_max;
-          nu_max = own ? nmax(nu_max, double(nu)) : nu_max;
-        }
-      } else {
-        // Compensated update: U0 = hi + lo (two FP32 words).  Stage states
-        // are the FP32 values hi + d with an FP32 increment d (lo, below half
-        // an ulp of hi, changes such a value by at most its own rounding, so
-        // the other stages do not read it); the last stage adds its
-        // increment to hi + lo in FP64 and splits the sum back into (hi, lo).
-        // The SSP middle stage forms its state as its input (hi + d1, the
-        // window's zp) + (d2 - d1), without reading hi: within about an ulp
-        // of hi + d2, the FP32 state's own precision (the increments, and so
-        // the FP64 accumulation, are unchanged).
-        R* lo = static_cast<R*>(a.lo);
-        R* dv = static_cast<R*>(a.dinc);
-        R* sn = static_cast<R*>(a.snext);
-        const bool fin = FIN == 2 ? a.comp >= MFVM_COMP_FINAL_SET : FIN == 1;
-        const bool needd = a.comp == MFVM_COMP_SSP || a.comp == MFVM_COMP_FINAL_SSP;
-        const bool needk = a.comp == MFVM_COMP_FINAL_ACC6 || a.acc == MFVM_ACC_ADD;
-        const R* U0 = static_cast<const R*>(a.U0);
-        double nw[5];
-        R hv[5], lv[5], dvv[5], kav[5];
-        const bool needh = fin || a.comp != MFVM_COMP_SSP;
-#pragma unroll
-        for (int q = 0; q < 5; ++q) {
-          const long long og = oc + q * fs, ii = oi + q * a.il.fs;
-          hv[q] = (u0_own || !needh) ? zp[q] : U0[og];
-          lv[q] = fin ? lo[og] : R(0);
-          dvv[q] = needd ? dv[ii] : R(0);
-          kav[q] = needk ? ka[ii] : R(0);
-        }
-#pragma unroll
-        for (int q = 0; q < 5; ++q) {
-          const R kq = kr[q];
-          const long long og = oc + q * fs, ii = oi + q * a.il.fs;
-          const R h = hv[q], l = lv[q];
-          if (!fin) {
-            R d, nxt;
-            if (a.comp == MFVM_COMP_SET) {
-              d = R(a.c0 * dt) * kq;
-              nxt = h + d;
-            } else {
-              d = R(a.c0) * (dvv[q] + R(dt) * kq);
-              nxt = h + (d - dvv[q]);  // h: the stage input (see above)
-            }
-            if (own) {
-              dv[ii] = d;
-              sn[og] = nxt;
+        const long long oc = a.lay.at(ic, jc, k);
+        const long long oi = a.il.at(ic, jc, k);
+        R* out = static_cast<R*>(a.out);
+        R* ka = static_cast<R*>(a.kacc);
+        // The cell's reads of U0, kacc, lo and dinc are all issued before its
+        // first store (the arrays do not alias the outputs), so that they take
+        // one memory round trip instead of one per field.
+        if constexpr (!COMP) {
+          R nu_[5], u0v[5], kav[5];
+          const R dtr = R(dt);
+          if constexpr (EARLY) {
+  #pragma unroll
+            for (int q = 0; q < 5; ++q) {
+              u0v[q] = u0_own ? zp[q] : f0[q];
+              kav[q] = f1[q];
             }
           } else {
-            double inc;
-            if (a.comp == MFVM_COMP_FINAL_SET) inc = (a.c0 * dt) * double(kq);
-            else if (a.comp == MFVM_COMP_FINAL_SSP) inc = a.c0 * (double(dvv[q]) + dt * double(kq));
-            else inc = (dt / 6.0) * (double(kav[q]) + double(kq));
-            const double v = (double(h) + double(l)) + inc;
-            const R nh = R(v);
-            if (own) {
-              out[og] = nh;
-              lo[og] = R(v - double(nh));
+            const R* U0 = static_cast<const R*>(a.U0);
+            const bool need0 = a.update == MFVM_UPD_BASE || a.update == MFVM_UPD_SSP || a.update == MFVM_UPD_ACC6;
+            const bool needk = a.update == MFVM_UPD_ACC6 || a.acc == MFVM_ACC_ADD;
+  #pragma unroll
+            for (int q = 0; q < 5; ++q) {
+              u0v[q] = u0_own ? zp[q] : (need0 ? U0[oc + q * fs] : R(0));
+              kav[q] = needk ? ka[oi + q * a.il.fs] : R(0);
             }
-            nw[q] = v;
           }
-          if (own) {
-            if (a.acc == MFVM_ACC_SET) ka[ii] = kq;
-            else if (a.acc == MFVM_ACC_ADD) ka[ii] = kav[q] + R(a.acc_coef) * kq;
+  #pragma unroll
+          for (int q = 0; q < 5; ++q) {
+            const R uq = zp[q];  // stage input U at cell m-1
+            R o = uq;
+            switch (a.update) {
+              case MFVM_UPD_EULER: o = uq + dtr * kr[q]; break;
+              case MFVM_UPD_BASE: o = u0v[q] + R(a.c0 * dt) * kr[q]; break;
+              case MFVM_UPD_SSP: o = (R(a.c0) * u0v[q]) + (R(a.c1) * (uq + dtr * kr[q])); break;
+              case MFVM_UPD_ACC6: o = u0v[q] + R(dt / 6.0) * (kav[q] + kr[q]); break;
+              default: break;
+            }
+            nu_[q] = o;
+            if (own) {
+              if (a.update != MFVM_UPD_NONE) out[oc + q * fs] = o;
+              if (a.acc == MFVM_ACC_SET) ka[oi + q * a.il.fs] = kr[q];
+              else if (a.acc == MFVM_ACC_ADD) ka[oi + q * a.il.fs] = kav[q] + R(a.acc_coef) * kr[q];
+            }
+          }
+          if (a.partials) {
+            R rate, nu;
+            cell_cfl<UNIF, R>(a.geo, a.p, ic, jc, k, nu_, rate, nu);
+            rate_max = own ? nmax(rate_max, double(rate)) : rate_max;
+            nu_max = own ? nmax(nu_max, double(nu)) : nu_max;
+          }
+        } else {
+          // Compensated update: U0 = hi + lo (two FP32 words).  Stage states
+          // are the FP32 values hi + d with an FP32 increment d (lo, below half
+          // an ulp of hi, changes such a value by at most its own rounding, so
+          // the other stages do not read it); the last stage adds its
+          // increment to hi + lo in FP64 and splits the sum back into (hi, lo).
+          // The SSP middle stage forms its state as its input (hi + d1, the
+          // window's zp) + (d2 - d1), without reading hi: within about an ulp
+          // of hi + d2, the FP32 state's own precision (the increments, and so
+          // the FP64 accumulation, are unchanged).
+          R* lo = static_cast<R*>(a.lo);
+          R* dv = static_cast<R*>(a.dinc);
+          R* sn = static_cast<R*>(a.snext);
+          const bool fin = FIN == 2 ? a.comp >= MFVM_COMP_FINAL_SET : FIN == 1;
+          const bool needd = a.comp == MFVM_COMP_SSP || a.comp == MFVM_COMP_FINAL_SSP;
+          const bool needk = a.comp == MFVM_COMP_FINAL_ACC6 || a.acc == MFVM_ACC_ADD;
+          const R* U0 = static_cast<const R*>(a.U0);
+          double nw[5];
+          R hv[5], lv[5], dvv[5], kav[5];
+          const bool needh = fin || a.comp != MFVM_COMP_SSP;
+  #pragma unroll
+          for (int q = 0; q < 5; ++q) {
+            const long long og = oc + q * fs, ii = oi + q * a.il.fs;
+            hv[q] = (u0_own || !needh) ? zp[q] : U0[og];
+            lv[q] = fin ? lo[og] : R(0);
+            dvv[q] = needd ? dv[ii] : R(0);
+            kav[q] = needk ? ka[ii] : R(0);
+          }
+  #pragma unroll
+          for (int q = 0; q < 5; ++q) {
+            const R kq = kr[q];
+            const long long og = oc + q * fs, ii = oi + q * a.il.fs;
+            const R h = hv[q], l = lv[q];
+            if (!fin) {
+              R d, nxt;
+              if (a.comp == MFVM_COMP_SET) {
+                d = R(a.c0 * dt) * kq;
+                nxt = h + d;
+              } else {
+                d = R(a.c0) * (dvv[q] + R(dt) * kq);
+                nxt = h + (d - dvv[q]);  // h: the stage input (see above)
+              }
+              if (own) {
+                dv[ii] = d;
+                sn[og] = nxt;
+              }
+            } else {
+              double inc;
+              if (a.comp == MFVM_COMP_FINAL_SET) inc = (a.c0 * dt) * double(kq);
+              else if (a.comp == MFVM_COMP_FINAL_SSP) inc = a.c0 * (double(dvv[q]) + dt * double(kq));
+              else inc = (dt / 6.0) * (double(kav[q]) + double(kq));
+              const double v = (double(h) + double(l)) + inc;
+              const R nh = R(v);
+              if (own) {
+                out[og] = nh;
+                lo[og] = R(v - double(nh));
+              }
+              nw[q] = v;
+            }
+            if (own) {
+              if (a.acc == MFVM_ACC_SET) ka[ii] = kq;
+              else if (a.acc == MFVM_ACC_ADD) ka[ii] = kav[q] + R(a.acc_coef) * kq;
+            }
+          }
+          if (fin && a.partials) {
+            double rate, nu;
+            cell_cfl<UNIF, double>(a.geo, a.p, ic, jc, k, nw, rate, nu);
+            rate_max = own ? nmax(rate_max, rate) : rate_max;
+            nu_max = own ? nmax(nu_max, nu) : nu_max;
           }
         }
-        if (fin && a.partials) {
-          double rate, nu;
-          cell_cfl<UNIF, double>(a.geo, a.p, ic, jc, k, nw, rate, nu);
-          rate_max = own ? nmax(rate_max, rate) : rate_max;
-          nu_max = own ? nmax(nu_max, nu) : nu_max;
+      }
+
+      // ---- rotate the window -------------------------------------------------------
+      {
+  #pragma unroll
+        for (int q = 0; q < 5; ++q) fsz_prev[q] = fsz[q];
+  #pragma unroll
+        for (int f = 0; f < NQ; ++f) lz_prev[f] = lz_cur[f];
+        if (!LATE && m < ke) {
+  #pragma unroll
+          for (int q = 0; q < 5; ++q) {
+            acc_prev[q] = accm[q];
+            src_prev[q] = srcm[q];
+          }
+          src_any_prev = src_any;
+        }
+        zs(cu, zp);
+  #pragma unroll
+        for (int f = 0; f < 12; ++f) cu[f] = nx[f];
+        if constexpr (ZONLY) {
+  #pragma unroll
+          for (int f = 0; f < 12; ++f) nx[f] = nn[f];
+        }
+      }
+    };
+    if constexpr (COMP) {
+      // close(m-1) before face(m): no store is issued between a plane's loads
+      // and the end of the trip, where the carried window is handed over, so
+      // that hand-over waits for those loads only, not for the stores to
+      // complete (compensated x3 pass at RT 256^3: 1.57 -> 1.53 ms per step;
+      // the float64 pass measured 1% slower in this order and keeps the other).
+      face(kb);
+      for (int m = kb + 1; m <= ke; ++m) {
+        close(m - 1);
+        face(m);
+      }
+      close(ke);
+    } else {
+      for (int m = kb; m <= ke; ++m) {
+        face(m);
+        close(m);
+      }
+    }
+
+  } else {
+    for (int m = kb; m <= ke; ++m) {
+      // ---- x3 face m-1/2 -------------------------------------------------------
+      R fsz[5];
+      R lz_cur[NQ];
+      R nn[12];  // ZONLY: plane m+2, loaded now for the next iteration
+      // EARLY: the reads of the cell finalized in this iteration (m-1) -- U0,
+      // kacc and, in the LATE form, the in-plane divergence -- are issued here,
+      // before the x3 face's work, so that their latency overlaps it (the
+      // float64 x3-only pass stays at 2 waves per SIMD).  The other forms issue
+      // them at the finalize (early reads would cost the compensated x3 pass a
+      // wave per SIMD).
+      R fxy[5], f0[5], f1[5];
+      auto xy_reads = [&](int k) {
+        const long long oi = a.il.at(ic, jc, k);
+  #pragma unroll
+        for (int q = 0; q < 5; ++q) fxy[q] = static_cast<const R*>(a.xy)[oi + q * a.il.fs];
+      };
+      // (U0 == U, as in the first stage of a step: U0 of cell m-1 is the
+      // window's zp, read already)
+      const bool u0_own = a.U0 == a.U;
+      auto base_reads = [&](int k) {
+        const long long oc = a.lay.at(ic, jc, k), oi = a.il.at(ic, jc, k);
+        const bool need0 = !u0_own && (a.update == MFVM_UPD_BASE || a.update == MFVM_UPD_SSP || a.update == MFVM_UPD_ACC6);
+        const bool needk = a.update == MFVM_UPD_ACC6 || a.acc == MFVM_ACC_ADD;
+  #pragma unroll
+        for (int q = 0; q < 5; ++q) {
+          f0[q] = need0 ? static_cast<const R*>(a.U0)[oc + q * fs] : R(0);
+          f1[q] = needk ? static_cast<const R*>(a.kacc)[oi + q * a.il.fs] : R(0);
+        }
+      };
+      if constexpr (EARLY) {
+        if (m > kb) {
+          if constexpr (LATE) xy_reads(m - 1);
+          base_reads(m - 1);
+        }
+      }
+      {
+        R zc[NQ], lo[NQ];
+        zs(cu, zc);
+        if constexpr (ZONLY) {
+          if (m + 2 <= n3 + 1 && m + 1 <= ke) load12(m + 2, nn);
+        } else {
+          if (m + 1 <= n3 + 1) load12(m + 1, nx);
+        }
+        if (m == n3 + 1 || LIM == MFVM_LIM_NONE) {
+  #pragma unroll
+          for (int f = 0; f < NQ; ++f) lo[f] = lz_cur[f] = zc[f];
+        } else {
+          R zn[NQ];
+          zs(nx, zn);
+          slope_states<2, LIM, UNIF, R>(a.geo, ic, jc, m, visc ? 9 : 6, zp, zc, zn, lo, lz_cur);
+        }
+        const R s3 = g_area<UNIF, R>(a.geo, 2, ic - 1, jc - 1, m - 1);
+        face_flux<2, RS, R>(a.p, lz_prev, lo, s3, fsz);
+      }
+
+      // ---- in-plane faces of plane m (all threads take part in the barriers) ----
+      // Phase 1: x1 and x2 states of the own cells and of the tile-edge cells;
+      // phase 2: F.S of every x1 and x2 face of the tile, each computed once;
+      // phase 3: flux differences.  Helper tasks go to the first threads in
+      // phase 1 and to the last threads in phase 2, so that no wave takes two.
+      R accm[5], srcm[5];
+      bool src_any = false;
+      if (!LATE && m < ke) {
+        if constexpr (ZONLY) {
+          const long long oi = a.il.at(ic, jc, m);
+  #pragma unroll
+          for (int q = 0; q < 5; ++q) accm[q] = static_cast<const R*>(a.xy)[oi + q * a.il.fs];
+        } else {
+          inplane_div<LIM, RS, UNIF, R>(a, src, sm, BX, BY, tx, ty, tid, i0, j0, i, j, own, m, cu, accm);
+        }
+        cell_source<UNIF, R>(a.geo, a.p, ic, jc, m, cu, cu + 5, srcm, src_any);
+      }
+
+      // ---- finalize cell m-1 (stores from the lanes of interior columns) --------
+      if (m > kb) {
+        const int k = m - 1;
+        R kr[5];
+        R vol;
+        const bool vp2 = a.geo.vpow2 != 0;
+        if (!vp2) vol = g_vol<UNIF, R>(a.geo, ic, jc, k);
+        if constexpr (LATE) {
+          if constexpr (EARLY) {
+  #pragma unroll
+            for (int q = 0; q < 5; ++q) acc_prev[q] = fxy[q];
+          } else {
+            const long long oi = a.il.at(ic, jc, k);
+  #pragma unroll
+            for (int q = 0; q < 5; ++q) acc_prev[q] = static_cast<const R*>(a.xy)[oi + q * a.il.fs];
+          }
+          // (uniform Cartesian: the source reads U only, never the closure)
+          cell_source<UNIF, R>(a.geo, a.p, ic, jc, k, zp, zp + 5, src_prev, src_any_prev);
+        }
+  #pragma unroll
+        for (int q = 0; q < 5; ++q) {
+          const R x = -((acc_prev[q] + fsz[q]) - fsz_prev[q]);
+          kr[q] = vp2 ? x * R(a.geo.vinv) : qdiv(x, vol);
+          if (src_any_prev) kr[q] = kr[q] + src_prev[q];
+        }
+        const long long oc = a.lay.at(ic, jc, k);
+        const long long oi = a.il.at(ic, jc, k);
+        R* out = static_cast<R*>(a.out);
+        R* ka = static_cast<R*>(a.kacc);
+        // The cell's reads of U0, kacc, lo and dinc are all issued before its
+        // first store (the arrays do not alias the outputs), so that they take
+        // one memory round trip instead of one per field.
+        if constexpr (!COMP) {
+          R nu_[5], u0v[5], kav[5];
+          const R dtr = R(dt);
+          if constexpr (EARLY) {
+  #pragma unroll
+            for (int q = 0; q < 5; ++q) {
+              u0v[q] = u0_own ? zp[q] : f0[q];
+              kav[q] = f1[q];
+            }
+          } else {
+            const R* U0 = static_cast<const R*>(a.U0);
+            const bool need0 = a.update == MFVM_UPD_BASE || a.update == MFVM_UPD_SSP || a.update == MFVM_UPD_ACC6;
+            const bool needk = a.update == MFVM_UPD_ACC6 || a.acc == MFVM_ACC_ADD;
+  #pragma unroll
+            for (int q = 0; q < 5; ++q) {
+              u0v[q] = u0_own ? zp[q] : (need0 ? U0[oc + q * fs] : R(0));
+              kav[q] = needk ? ka[oi + q * a.il.fs] : R(0);
+            }
+          }
+  #pragma unroll
+          for (int q = 0; q < 5; ++q) {
+            const R uq = zp[q];  // stage input U at cell m-1
+            R o = uq;
+            switch (a.update) {
+              case MFVM_UPD_EULER: o = uq + dtr * kr[q]; break;
+              case MFVM_UPD_BASE: o = u0v[q] + R(a.c0 * dt) * kr[q]; break;
+              case MFVM_UPD_SSP: o = (R(a.c0) * u0v[q]) + (R(a.c1) * (uq + dtr * kr[q])); break;
+              case MFVM_UPD_ACC6: o = u0v[q] + R(dt / 6.0) * (kav[q] + kr[q]); break;
+              default: break;
+            }
+            nu_[q] = o;
+            if (own) {
+              if (a.update != MFVM_UPD_NONE) out[oc + q * fs] = o;
+              if (a.acc == MFVM_ACC_SET) ka[oi + q * a.il.fs] = kr[q];
+              else if (a.acc == MFVM_ACC_ADD) ka[oi + q * a.il.fs] = kav[q] + R(a.acc_coef) * kr[q];
+            }
+          }
+          if (a.partials) {
+            R rate, nu;
+            cell_cfl<UNIF, R>(a.geo, a.p, ic, jc, k, nu_, rate, nu);
+            rate_max = own ? nmax(rate_max, double(rate)) : rate_max;
+            nu_max = own ? nmax(nu_max, double(nu)) : nu_max;
+          }
+        } else {
+          // Compensated update: U0 = hi + lo (two FP32 words).  Stage states
+          // are the FP32 values hi + d with an FP32 increment d (lo, below half
+          // an ulp of hi, changes such a value by at most its own rounding, so
+          // the other stages do not read it); the last stage adds its
+          // increment to hi + lo in FP64 and splits the sum back into (hi, lo).
+          // The SSP middle stage forms its state as its input (hi + d1, the
+          // window's zp) + (d2 - d1), without reading hi: within about an ulp
+          // of hi + d2, the FP32 state's own precision (the increments, and so
+          // the FP64 accumulation, are unchanged).
+          R* lo = static_cast<R*>(a.lo);
+          R* dv = static_cast<R*>(a.dinc);
+          R* sn = static_cast<R*>(a.snext);
+          const bool fin = FIN == 2 ? a.comp >= MFVM_COMP_FINAL_SET : FIN == 1;
+          const bool needd = a.comp == MFVM_COMP_SSP || a.comp == MFVM_COMP_FINAL_SSP;
+          const bool needk = a.comp == MFVM_COMP_FINAL_ACC6 || a.acc == MFVM_ACC_ADD;
+          const R* U0 = static_cast<const R*>(a.U0);
+          double nw[5];
+          R hv[5], lv[5], dvv[5], kav[5];
+          const bool needh = fin || a.comp != MFVM_COMP_SSP;
+  #pragma unroll
+          for (int q = 0; q < 5; ++q) {
+            const long long og = oc + q * fs, ii = oi + q * a.il.fs;
+            hv[q] = (u0_own || !needh) ? zp[q] : U0[og];
+            lv[q] = fin ? lo[og] : R(0);
+            dvv[q] = needd ? dv[ii] : R(0);
+            kav[q] = needk ? ka[ii] : R(0);
+          }
+  #pragma unroll
+          for (int q = 0; q < 5; ++q) {
+            const R kq = kr[q];
+            const long long og = oc + q * fs, ii = oi + q * a.il.fs;
+            const R h = hv[q], l = lv[q];
+            if (!fin) {
+              R d, nxt;
+              if (a.comp == MFVM_COMP_SET) {
+                d = R(a.c0 * dt) * kq;
+                nxt = h + d;
+              } else {
+                d = R(a.c0) * (dvv[q] + R(dt) * kq);
+                nxt = h + (d - dvv[q]);  // h: the stage input (see above)
+              }
+              if (own) {
+                dv[ii] = d;
+                sn[og] = nxt;
+              }
+            } else {
+              double inc;
+              if (a.comp == MFVM_COMP_FINAL_SET) inc = (a.c0 * dt) * double(kq);
+              else if (a.comp == MFVM_COMP_FINAL_SSP) inc = a.c0 * (double(dvv[q]) + dt * double(kq));
+              else inc = (dt / 6.0) * (double(kav[q]) + double(kq));
+              const double v = (double(h) + double(l)) + inc;
+              const R nh = R(v);
+              if (own) {
+                out[og] = nh;
+                lo[og] = R(v - double(nh));
+              }
+              nw[q] = v;
+            }
+            if (own) {
+              if (a.acc == MFVM_ACC_SET) ka[ii] = kq;
+              else if (a.acc == MFVM_ACC_ADD) ka[ii] = kav[q] + R(a.acc_coef) * kq;
+            }
+          }
+          if (fin && a.partials) {
+            double rate, nu;
+            cell_cfl<UNIF, double>(a.geo, a.p, ic, jc, k, nw, rate, nu);
+            rate_max = own ? nmax(rate_max, rate) : rate_max;
+            nu_max = own ? nmax(nu_max, nu) : nu_max;
+          }
+        }
+      }
+
+      // ---- rotate the window -------------------------------------------------------
+      {
+  #pragma unroll
+        for (int q = 0; q < 5; ++q) fsz_prev[q] = fsz[q];
+  #pragma unroll
+        for (int f = 0; f < NQ; ++f) lz_prev[f] = lz_cur[f];
+        if (!LATE && m < ke) {
+  #pragma unroll
+          for (int q = 0; q < 5; ++q) {
+            acc_prev[q] = accm[q];
+            src_prev[q] = srcm[q];
+          }
+          src_any_prev = src_any;
+        }
+        zs(cu, zp);
+  #pragma unroll
+        for (int f = 0; f < 12; ++f) cu[f] = nx[f];
+        if constexpr (ZONLY) {
+  #pragma unroll
+          for (int f = 0; f < 12; ++f) nx[f] = nn[f];
         }
       }
     }
 
-    // ---- rotate the window -------------------------------------------------------
-    {
-#pragma unroll
-      for (int q = 0; q < 5; ++q) fsz_prev[q] = fsz[q];
-#pragma unroll
-      for (int f = 0; f < NQ; ++f) lz_prev[f] = lz_cur[f];
-      if (!LATE && m < ke) {
-#pragma unroll
-        for (int q = 0; q < 5; ++q) {
-          acc_prev[q] = accm[q];
-          src_prev[q] = srcm[q];
-        }
-        src_any_prev = src_any;
-      }
-      zs(cu, zp);
-#pragma unroll
-      for (int f = 0; f < 12; ++f) cu[f] = nx[f];
-      if constexpr (ZONLY) {
-#pragma unroll
-        for (int f = 0; f < 12; ++f) nx[f] = nn[f];
-      }
-    }
-  };
-  if constexpr (COMP) {
-    // close(m-1) before face(m): no store is issued between a plane's loads
-    // and the end of the trip, where the carried window is handed over, so
-    // that hand-over waits for those loads only, not for the stores to
-    // complete (compensated x3 pass at RT 256^3: 1.57 -> 1.53 ms per step;
-    // the float64 pass measured 1% slower in this order and keeps the other).
-    face(kb);
-    for (int m = kb + 1; m <= ke; ++m) {
-      close(m - 1);
-      face(m);
-    }
-    close(ke);
-  } else {
-    for (int m = kb; m <= ke; ++m) {
-      face(m);
-      close(m);
-    }
   }
 
   if (tl.slot) block_max2<double>(rate_max, nu_max, tl.slot, reinterpret_cast<double*>(smem_raw));
