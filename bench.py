@@ -43,8 +43,8 @@ XY_OWN_BYTES_PER_CELL = 128.0
 # FETCH_SIZE * 1024 * 2.0, written = WRITE_SIZE * 1024 * 1.0).  The profiled
 # kernel build is named with it; None where no such profile exists.
 PROFILED_TRAFFIC = {
-    ("ns_xyrow_kernel", 256): ((1129063 * 2.0 + 655360 * 1.0) * 1024,
-                               "profiles/ns_xyrow_kernel_rt256.md: prof_rt FETCH_SIZE / WRITE_SIZE runs"),
+    ("ns_xyrow_kernel", 256): ((1024564 * 2.0 + 655360 * 1.0) * 1024,
+                               "profiles/ns_xyrow_kernel_rt256.md: prof_rt4 FETCH_SIZE / WRITE_SIZE runs"),
 }
 
 
