@@ -12,13 +12,15 @@
 
 namespace mfvm {
 
-template <class R>
+// I32: the shell has fewer than 2^31 cells (32-bit index arithmetic).
+template <class R, bool I32>
 __global__ void bc_kernel(BCArgs a, const R* __restrict__ src, R* __restrict__ dst, long long nshell) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int var = blockIdx.y;
   if (t >= nshell) return;
   int i, j, k;
-  shell_cell(a.lay, t, i, j, k);
+  if constexpr (I32) shell_cell32(a.lay, (unsigned)t, i, j, k);
+  else shell_cell(a.lay, t, i, j, k);
   const R* f = src + var * a.lay.fs;
   dst[var * a.lay.fs + a.lay.at(i, j, k)] = shell_value<R>(a, var, f, i, j, k);
 }
@@ -46,8 +48,12 @@ hipError_t launch_bcs(const mfvm_grid* g, int nvar, const int32_t* bc, const voi
   const long long nshell = shell_count(g->n[0], g->n[1], g->n[2]);
   const dim3 grd((unsigned)((nshell + 255) / 256), nvar, 1);
   const int tk = t_begin(MFVM_TK_BC, s);
-  hipLaunchKernelGGL(bc_kernel<R>, grd, dim3(256), 0, s, a, static_cast<const R*>(src),
-                     static_cast<R*>(dst), nshell);
+  if (nshell < (1ll << 31))
+    hipLaunchKernelGGL((bc_kernel<R, true>), grd, dim3(256), 0, s, a, static_cast<const R*>(src),
+                       static_cast<R*>(dst), nshell);
+  else
+    hipLaunchKernelGGL((bc_kernel<R, false>), grd, dim3(256), 0, s, a, static_cast<const R*>(src),
+                       static_cast<R*>(dst), nshell);
   t_end(tk, s);
   return hipGetLastError();
 }

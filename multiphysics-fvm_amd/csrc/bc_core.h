@@ -93,6 +93,35 @@ MFVM_DEV void shell_cell(const Layout& lay, long long t, int& i, int& j, int& k)
   }
 }
 
+// The same cell in 32-bit arithmetic, for shells below 2^31 cells (one
+// unsigned division per coordinate pair instead of several 64-bit ones).
+MFVM_DEV void shell_cell32(const Layout& lay, unsigned t, int& i, int& j, int& k) {
+  const unsigned n1 = lay.n1, n2 = lay.n2, n3 = lay.n3;
+  const unsigned N2 = n2 + 2, N3 = n3 + 2;
+  const unsigned pA = N2 * N3, pB = n1 * N3;
+  if (t < 2 * pA) {
+    const bool hi = t >= pA;
+    const unsigned r = hi ? t - pA : t, q = r / N2;
+    i = hi ? (int)n1 + 1 : 0;
+    j = (int)(r - q * N2);
+    k = (int)q;
+  } else if (t < 2 * pA + 2 * pB) {
+    const unsigned u = t - 2 * pA;
+    const bool hi = u >= pB;
+    const unsigned r = hi ? u - pB : u, q = r / n1;
+    j = hi ? (int)n2 + 1 : 0;
+    i = 1 + (int)(r - q * n1);
+    k = (int)q;
+  } else {
+    const unsigned u = t - 2 * pA - 2 * pB, plane = n1 * n2;
+    const bool hi = u >= plane;
+    const unsigned r = hi ? u - plane : u, q = r / n1;
+    k = hi ? (int)n3 + 1 : 0;
+    i = 1 + (int)(r - q * n1);
+    j = 1 + (int)q;
+  }
+}
+
 inline long long shell_count(int n1, int n2, int n3) {
   return (long long)(n1 + 2) * (n2 + 2) * (n3 + 2) - (long long)n1 * n2 * n3;
 }

@@ -62,22 +62,35 @@ MFVM_DEV long long ens_item(long long v, long long n, bool remap) {
   return (long long)(x * q + (x < r ? x : r) + ((unsigned long long)v >> 3));
 }
 
-template <class R>
+// One thread per ghost cell and field of every member.  I32: fewer than 2^31
+// such threads in all (32-bit index arithmetic).  The member is found once
+// per wave, by a binary search on the wave's first thread, in scalar loads;
+// a lane past that member's range (a wave that straddles two members) steps
+// on from there.
+template <class R, bool I32>
 __global__ __launch_bounds__(kEnsBlock) void ens_bc_kernel(const EnsMem* __restrict__ mem, int nm, int src_sel,
                                                            int dst_sel, long long total) {
   const long long t = (long long)blockIdx.x * kEnsBlock + threadIdx.x;
-  if (t >= total) return;
+  const long long t0 = (long long)blockIdx.x * kEnsBlock + __builtin_amdgcn_readfirstlane(threadIdx.x & ~63u);
   int lo = 0, hi = nm - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (mem[mid].bc_off <= t) lo = mid;
+    if (mem[mid].bc_off <= t0) lo = mid;
     else hi = mid - 1;
   }
+  if (t >= total) return;
+  while (lo + 1 < nm && mem[lo + 1].bc_off <= t) ++lo;
   const EnsMem& e = mem[lo];
-  const long long u = t - e.bc_off;
-  const int var = (int)(u / e.nshell);
-  int i, j, k;
-  shell_cell(e.lay, u % e.nshell, i, j, k);
+  int var, i, j, k;
+  if constexpr (I32) {
+    const unsigned u = (unsigned)(t - e.bc_off), ns = (unsigned)e.nshell, v = u / ns;
+    var = (int)v;
+    shell_cell32(e.lay, u - v * ns, i, j, k);
+  } else {
+    const long long u = t - e.bc_off;
+    var = (int)(u / e.nshell);
+    shell_cell(e.lay, u % e.nshell, i, j, k);
+  }
   const R* src = static_cast<const R*>(e.buf[src_sel]) + var * e.lay.fs;
   R* dst = static_cast<R*>(e.buf[dst_sel]);
   dst[var * e.lay.fs + e.lay.at(i, j, k)] = shell_value<R>(e.bca, var, src, i, j, k);
@@ -474,7 +487,10 @@ static void launch_ens_stage(const EnsRun& r, int q, hipStream_t s) {
 template <class R>
 static void launch_ens_bc(const EnsRun& r, int src, int dst, hipStream_t s) {
   const unsigned g = (unsigned)((r.bc_total + kEnsBlock - 1) / kEnsBlock);
-  hipLaunchKernelGGL(ens_bc_kernel<R>, dim3(g), dim3(kEnsBlock), 0, s, r.d_mem, r.n, src, dst, r.bc_total);
+  if (r.bc_total < (1ll << 31))
+    hipLaunchKernelGGL((ens_bc_kernel<R, true>), dim3(g), dim3(kEnsBlock), 0, s, r.d_mem, r.n, src, dst, r.bc_total);
+  else
+    hipLaunchKernelGGL((ens_bc_kernel<R, false>), dim3(g), dim3(kEnsBlock), 0, s, r.d_mem, r.n, src, dst, r.bc_total);
 }
 
 template <class R>
