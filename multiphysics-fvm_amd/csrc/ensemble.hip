@@ -1,8 +1,8 @@
 // Batched ensemble of Navier-Stokes instances (mfvm_ens_advance).
 //
 // Every member advances with exactly the arithmetic of the single-grid path
-// (the same device bodies: closure_march_body, ns_stage_body, cfl_tile_body,
-// shell_value), so each member's result equals its own mfvm_ns_advance /
+// (the same device bodies: closure_march_body, ns_stage_body, ns_xyrow_strip,
+// cfl_tile_body, shell_value), so each member's result equals its own mfvm_ns_advance /
 // mfvm_ns_comp_advance run bit for bit.  What changes is the launch shape:
 // each phase of a stage (ghost fill, closure, the fused stage or the split
 // x1/x2 and x3 passes) is ONE launch over a work list holding the tiles of
@@ -13,7 +13,8 @@
 // replayed.
 //
 // Device table (caller-owned, mfvm_ens_table_bytes):
-//   EnsMem[n] | StageArgs[nstages][n] | stage work | closure work | dt[n] | step[n]
+//   EnsMem[n] | StageArgs[nstages][n] | stage work | closure work |
+//   x1/x2 row-march work | dt[n] | step[n]
 // dt[m] is the member's current dt; step[m] counts the dts written to
 // m.dts (the finalize kernel appends one per step), so one captured step is
 // valid for every replay.
@@ -64,36 +65,42 @@ MFVM_DEV long long ens_item(long long v, long long n, bool remap) {
 
 // One thread per ghost cell and field of every member.  I32: fewer than 2^31
 // such threads in all (32-bit index arithmetic).  The member is found once
-// per wave, by a binary search on the wave's first thread, in scalar loads;
-// a lane past that member's range (a wave that straddles two members) steps
-// on from there.
+// per wave, by a binary search on the wave's first thread; a wave that
+// straddles members takes them one after another.  The member index stays
+// wave-uniform, so every read of the member table is a scalar load.
 template <class R, bool I32>
 __global__ __launch_bounds__(kEnsBlock) void ens_bc_kernel(const EnsMem* __restrict__ mem, int nm, int src_sel,
                                                            int dst_sel, long long total) {
   const long long t = (long long)blockIdx.x * kEnsBlock + threadIdx.x;
   const long long t0 = (long long)blockIdx.x * kEnsBlock + __builtin_amdgcn_readfirstlane(threadIdx.x & ~63u);
-  int lo = 0, hi = nm - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (mem[mid].bc_off <= t0) lo = mid;
+  if (t0 >= total) return;
+  int mb = 0, hi = nm - 1;
+  while (mb < hi) {
+    const int mid = (mb + hi + 1) >> 1;
+    if (mem[mid].bc_off <= t0) mb = mid;
     else hi = mid - 1;
   }
-  if (t >= total) return;
-  while (lo + 1 < nm && mem[lo + 1].bc_off <= t) ++lo;
-  const EnsMem& e = mem[lo];
-  int var, i, j, k;
-  if constexpr (I32) {
-    const unsigned u = (unsigned)(t - e.bc_off), ns = (unsigned)e.nshell, v = u / ns;
-    var = (int)v;
-    shell_cell32(e.lay, u - v * ns, i, j, k);
-  } else {
-    const long long u = t - e.bc_off;
-    var = (int)(u / e.nshell);
-    shell_cell(e.lay, u % e.nshell, i, j, k);
+  for (;;) {
+    const EnsMem& e = mem[mb];
+    const long long end = mb + 1 < nm ? mem[mb + 1].bc_off : total;
+    if (t >= e.bc_off && t < end) {
+      int var, i, j, k;
+      if constexpr (I32) {
+        const unsigned u = (unsigned)(t - e.bc_off), ns = (unsigned)e.nshell, v = u / ns;
+        var = (int)v;
+        shell_cell32(e.lay, u - v * ns, i, j, k);
+      } else {
+        const long long u = t - e.bc_off;
+        var = (int)(u / e.nshell);
+        shell_cell(e.lay, u % e.nshell, i, j, k);
+      }
+      const R* src = static_cast<const R*>(e.buf[src_sel]) + var * e.lay.fs;
+      R* dst = static_cast<R*>(e.buf[dst_sel]);
+      dst[var * e.lay.fs + e.lay.at(i, j, k)] = shell_value<R>(e.bca, var, src, i, j, k);
+    }
+    if (t0 + 63 < end || mb + 1 >= nm || end >= total) break;
+    ++mb;
   }
-  const R* src = static_cast<const R*>(e.buf[src_sel]) + var * e.lay.fs;
-  R* dst = static_cast<R*>(e.buf[dst_sel]);
-  dst[var * e.lay.fs + e.lay.at(i, j, k)] = shell_value<R>(e.bca, var, src, i, j, k);
 }
 
 // P2 / SPEC (these kernels): every member's geometry is uniform powers of
@@ -154,30 +161,36 @@ __global__ __launch_bounds__(kEnsBlock, 2) void ens_stage_comp_kernel(const EnsM
   ens_stage_loop<LIM, RS, UNIF, float, true, SPEC>(mem, st, work, nwork, remap);
 }
 
-// Split stage over the work list: the x1/x2 pass of every member's tiles,
-// then the x3 pass (as ns_xy_kernel / ns_z_kernel of the single-grid path).
-// The x1/x2 pass is held to four (float32) or three (float64) waves per SIMD
-// (unbounded, the float64 pass took 174 VGPRs and two waves).
-template <int LIM, int RS, bool UNIF, class R, int SPEC>
-__global__ __launch_bounds__(kEnsBlock, sizeof(R) == sizeof(float) ? 4 : 3) void ens_xy_kernel(const EnsMem* __restrict__ mem,
-                                                           const StageArgs* __restrict__ st,
-                                                           const EnsWork* __restrict__ work, long long nwork,
-                                                           int remap) {
-  for (long long v = blockIdx.x; v < nwork; v += gridDim.x) {
-    const EnsWork w = work[ens_item(v, nwork, remap != 0)];
-    const EnsMem& e = mem[w.m];
-    StageArgs a = st[w.m];
-    spec_args(a, SPEC);
-    TileCtx tc;
-    tc.bx = e.bx;
-    tc.by = e.by;
-    tc.x = w.t % e.ntx;
-    tc.y = (w.t / e.ntx) % e.nty;
-    tc.z = w.t / (e.ntx * e.nty);
-    tc.slot = nullptr;
-    ns_xy_body<LIM, RS, UNIF, R>(a, tc, ptr_src<R>(a));
-    __syncthreads();
-  }
+// Split stage over the work lists: the x1/x2 pass of every member, then the x3
+// pass over the members' tiles (as ns_xyrow_kernel / ns_z_kernel of the
+// single-grid path).  The x1/x2 pass is the row march (ns_xyrow_strip) over a
+// work list of wave tasks: task t of member m is strip t mod ns, row chunk
+// (t / ns) mod nc (ENS_XR_ROWS rows) and plane t / (ns nc) + 1.  Float64
+// reads P from the closure, float32 forms it from U.
+constexpr int ENS_XR_ROWS = 16;
+inline long long ens_xr_tasks(const mfvm_ens_member& m) {
+  const long long ns = (m.grid.n[0] + XR_W - 1) / XR_W, nc = (m.grid.n[1] + ENS_XR_ROWS - 1) / ENS_XR_ROWS;
+  return ns * nc * m.grid.n[2];
+}
+// BUF: every member's arrays are below 4 GiB (buf_src_ok): buffer-resource reads.
+template <int LIM, int RS, bool UNIF, class R, int SPEC, bool BUF>
+__global__ __launch_bounds__(XR_W * XR_WAVES, sizeof(R) == sizeof(float) ? 4 : 2) void ens_xyrow_kernel(
+    const EnsMem* __restrict__ mem, const StageArgs* __restrict__ st, const EnsWork* __restrict__ work,
+    long long nwork, int remap) {
+  extern __shared__ double smem_raw[];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / XR_W);
+  const long long v = (long long)blockIdx.x * XR_WAVES + wave;
+  if (v >= nwork) return;
+  const EnsWork w = work[ens_item(v, nwork, remap != 0)];
+  const EnsMem& e = mem[w.m];
+  StageArgs a = st[w.m];
+  spec_args(a, SPEC);
+  const int ns = (e.lay.n1 + XR_W - 1) / XR_W, nc = (e.lay.n2 + ENS_XR_ROWS - 1) / ENS_XR_ROWS;
+  const int sx = w.t % ns, c = (w.t / ns) % nc, m = 1 + w.t / (ns * nc);
+  const int jb = 1 + c * ENS_XR_ROWS, je = min(jb + ENS_XR_ROWS, e.lay.n2 + 1);
+  R* lds = reinterpret_cast<R*>(smem_raw) + wave * XR_LDS_ELEMS;
+  if constexpr (BUF) ns_xyrow_strip<LIM, RS, UNIF, R>(a, buf_src<R>(a), lds, 1 + sx * XR_W, jb, je, m);
+  else ns_xyrow_strip<LIM, RS, UNIF, R>(a, ptr_src<R>(a), lds, 1 + sx * XR_W, jb, je, m);
 }
 
 template <int LIM, int RS, bool UNIF, class R, bool COMP, int FIN = 2, int SPEC = 0>
@@ -345,16 +358,18 @@ static void closure_tiles(const mfvm_ens_member& m, int nt[3], int* kz) {
 }
 
 struct EnsLayout {
-  size_t off_mem, off_st, off_sw, off_cw, off_dt, off_step, bytes;
-  long long nsw, ncw;
+  size_t off_mem, off_st, off_sw, off_cw, off_xw, off_dt, off_step, bytes;
+  long long nsw, ncw, nxw;
 };
 
 static EnsLayout ens_layout(const mfvm_ens_member* m, int n, int nst) {
   EnsLayout L;
   L.nsw = 0;
   L.ncw = 0;
+  L.nxw = 0;
   for (int i = 0; i < n; ++i) {
     L.nsw += ens_member_tiles(&m[i]);
+    L.nxw += ens_xr_tasks(m[i]);
     int nt[3], kz;
     closure_tiles(m[i], nt, &kz);
     L.ncw += (long long)nt[0] * nt[1] * nt[2];
@@ -368,6 +383,8 @@ static EnsLayout ens_layout(const mfvm_ens_member* m, int n, int nst) {
   o = align256(o + sizeof(EnsWork) * L.nsw);
   L.off_cw = o;
   o = align256(o + sizeof(EnsWork) * L.ncw);
+  L.off_xw = o;
+  o = align256(o + sizeof(EnsWork) * L.nxw);
   L.off_dt = o;
   o = align256(o + sizeof(double) * n);
   L.off_step = o;
@@ -415,12 +432,14 @@ struct EnsRun {
   StageArgs* d_st;
   EnsWork* d_sw;
   EnsWork* d_cw;
-  long long nsw, ncw, bc_total;
+  EnsWork* d_xw;  // row-march wave tasks of the x1/x2 pass
+  long long nsw, ncw, nxw, bc_total;
   size_t lds_stage;
   int persist_blocks;  // 0: one workgroup per item
   int remap;
   bool split;          // every member has an xy scratch: x1/x2 pass + x3 pass
   bool p2;             // every member's geometry is uniform powers of two (pow2_geo)
+  bool buf;            // every member's arrays are below 4 GiB (buf_src_ok)
   int in_sel[4];
 };
 
@@ -434,9 +453,16 @@ static void launch_ens_stage_t(const EnsRun& r, const StageArgs* st, bool fin, h
   const unsigned g = grid_for(r.nsw, r.persist_blocks);
   const dim3 grd(g), blk(kEnsBlock);
   if (r.split) {
+    const dim3 xg((unsigned)((r.nxw + XR_WAVES - 1) / XR_WAVES)), xb(XR_W * XR_WAVES);
+#define MFVM_ENS_XYROW(RT)                                                                                   \
+    if (r.buf)                                                                                               \
+      hipLaunchKernelGGL((ens_xyrow_kernel<LIM, RS, UNIF, RT, SPEC, true>), xg, xb,                            \
+                         (size_t)XR_WAVES * XR_LDS_ELEMS * sizeof(RT), s, r.d_mem, st, r.d_xw, r.nxw, r.remap); \
+    else                                                                                                     \
+      hipLaunchKernelGGL((ens_xyrow_kernel<LIM, RS, UNIF, RT, SPEC, false>), xg, xb,                           \
+                         (size_t)XR_WAVES * XR_LDS_ELEMS * sizeof(RT), s, r.d_mem, st, r.d_xw, r.nxw, r.remap);
     if (r.comp) {
-      hipLaunchKernelGGL((ens_xy_kernel<LIM, RS, UNIF, float, SPEC>), grd, blk, r.lds_stage, s, r.d_mem, st, r.d_sw,
-                         r.nsw, r.remap);
+      MFVM_ENS_XYROW(float)
       if (fin)
         hipLaunchKernelGGL((ens_z_comp_kernel<LIM, RS, UNIF, true, SPEC>), grd, blk, r.lds_stage, s, r.d_mem, st,
                            r.d_sw, r.nsw, r.remap);
@@ -444,11 +470,11 @@ static void launch_ens_stage_t(const EnsRun& r, const StageArgs* st, bool fin, h
         hipLaunchKernelGGL((ens_z_comp_kernel<LIM, RS, UNIF, false, SPEC>), grd, blk, r.lds_stage, s, r.d_mem, st,
                            r.d_sw, r.nsw, r.remap);
     } else {
-      hipLaunchKernelGGL((ens_xy_kernel<LIM, RS, UNIF, double, SPEC>), grd, blk, r.lds_stage, s, r.d_mem, st, r.d_sw,
-                         r.nsw, r.remap);
+      MFVM_ENS_XYROW(double)
       hipLaunchKernelGGL((ens_z_kernel<LIM, RS, UNIF, SPEC>), grd, blk, r.lds_stage, s, r.d_mem, st, r.d_sw, r.nsw,
                          r.remap);
     }
+#undef MFVM_ENS_XYROW
     return;
   }
   if (r.comp)
@@ -567,16 +593,20 @@ hipError_t ens_advance(const mfvm_ens_member* m, int n, const mfvm_ns_params* p,
   r.cfl = cfl;
   r.unif = true;
   r.p2 = true;
+  r.buf = true;
   for (int i = 0; i < n; ++i) {
     r.unif = r.unif && m[i].grid.uniform;
     r.p2 = r.p2 && pow2_geo(make_geo(&m[i].grid));
+    r.buf = r.buf && buf_src_ok(make_layout(m[i].grid.n[0], m[i].grid.n[1], m[i].grid.n[2]), comp ? 4 : 8);
   }
   r.d_mem = reinterpret_cast<EnsMem*>(base + L.off_mem);
   r.d_st = reinterpret_cast<StageArgs*>(base + L.off_st);
   r.d_sw = reinterpret_cast<EnsWork*>(base + L.off_sw);
   r.d_cw = reinterpret_cast<EnsWork*>(base + L.off_cw);
+  r.d_xw = reinterpret_cast<EnsWork*>(base + L.off_xw);
   r.nsw = L.nsw;
   r.ncw = L.ncw;
+  r.nxw = L.nxw;
   r.remap = o->remap ? 1 : 0;
   r.split = true;
   for (int i = 0; i < n; ++i) r.split = r.split && m[i].xy != nullptr;
@@ -588,8 +618,8 @@ hipError_t ens_advance(const mfvm_ens_member* m, int n, const mfvm_ns_params* p,
   std::vector<char> host(L.bytes, 0);
   EnsMem* hm = reinterpret_cast<EnsMem*>(host.data() + L.off_mem);
   StageArgs* hs = reinterpret_cast<StageArgs*>(host.data() + L.off_st);
-  std::vector<long long> st_tiles(n), cl_tiles(n);
-  std::vector<double> st_cost(n), cl_cost(n);
+  std::vector<long long> st_tiles(n), cl_tiles(n), xr_tasks(n);
+  std::vector<double> st_cost(n), cl_cost(n), xr_cost(n);
   long long bc_off = 0;
   size_t lds = 0;
   for (int i = 0; i < n; ++i) {
@@ -632,6 +662,8 @@ hipError_t ens_advance(const mfvm_ens_member* m, int n, const mfvm_ns_params* p,
     st_cost[i] = e.kz;
     cl_tiles[i] = (long long)ct[0] * ct[1] * ct[2];
     cl_cost[i] = e.ckz;
+    xr_tasks[i] = ens_xr_tasks(mm);
+    xr_cost[i] = 1.0;
     lds = std::max(lds, stage_lds_bytes(e.bx, e.by, rsize));
     for (int q = 0; q < nst; ++q) {
       StageArgs a;
@@ -673,11 +705,13 @@ hipError_t ens_advance(const mfvm_ens_member* m, int n, const mfvm_ns_params* p,
   }
   r.bc_total = bc_off;
   r.lds_stage = lds;
-  std::vector<EnsWork> sw, cw;
+  std::vector<EnsWork> sw, cw, xw;
   ens_worklist(st_tiles, st_cost, o->order, sw);
   ens_worklist(cl_tiles, cl_cost, o->order, cw);
+  ens_worklist(xr_tasks, xr_cost, o->order, xw);
   memcpy(host.data() + L.off_sw, sw.data(), sizeof(EnsWork) * sw.size());
   memcpy(host.data() + L.off_cw, cw.data(), sizeof(EnsWork) * cw.size());
+  memcpy(host.data() + L.off_xw, xw.data(), sizeof(EnsWork) * xw.size());
 
   int dev = 0, ncu = 0;
   hipError_t e = hipGetDevice(&dev);

@@ -1447,45 +1447,6 @@ MFVM_DEV void ns_stage_body(const StageArgs& a, const TileCtx& tl, const S& src)
   if (tl.slot) block_max2<double>(rate_max, nu_max, tl.slot, reinterpret_cast<double*>(smem_raw));
 }
 
-// x1/x2 half of a split stage: the in-plane divergence of every cell of the
-// tile's planes, written to a.xy (interior layout).  Same arithmetic as the
-// fused body; the x3 half (ns_stage_body with ZONLY) completes the stage.
-template <int LIM, int RS, bool UNIF, class R, class S>
-MFVM_DEV void ns_xy_body(const StageArgs& a, const TileCtx& tl, const S& src) {
-  extern __shared__ double smem_raw[];
-  R* sm = reinterpret_cast<R*>(smem_raw);
-  const int BX = tl.bx, BY = tl.by;
-  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
-  const int tx = tid % BX, ty = tid / BX;
-  const int n1 = a.lay.n1, n2 = a.lay.n2, n3 = a.lay.n3;
-  const int i0 = 1 + tl.x * BX, j0 = 1 + tl.y * BY;
-  const int i = i0 + tx, j = j0 + ty;
-  const int kb = 1 + tl.z * a.kz;
-  const int ke = min(kb + a.kz, n3 + 1);
-  const bool own = (i <= n1) && (j <= n2);
-  const bool visc = a.p.visc != 0;
-  for (int m = kb; m < ke; ++m) {
-    R cu[12];
-    if (own) {
-      const long long o = a.lay.at(i, j, m);
-#pragma unroll
-      for (int f = 0; f < 5; ++f) cu[f] = src.u(o, f);
-#pragma unroll
-      for (int f = 0; f < NS_NC; ++f) cu[5 + f] = (f == 0 || visc) ? src.c(o, f) : R(0);
-    } else {
-#pragma unroll
-      for (int f = 0; f < 12; ++f) cu[f] = R(0);
-    }
-    R accm[5];
-    inplane_div<LIM, RS, UNIF, R>(a, src, sm, BX, BY, tx, ty, tid, i0, j0, i, j, own, m, cu, accm);
-    if (own) {
-      const long long oi = a.il.at(i, j, m);
-#pragma unroll
-      for (int q = 0; q < 5; ++q) static_cast<R*>(a.xy)[oi + q * a.il.fs] = accm[q];
-    }
-  }
-}
-
 // ---------------------------------------------------------------------------
 // Row-march x1/x2 pass
 // ---------------------------------------------------------------------------
