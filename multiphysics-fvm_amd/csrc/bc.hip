@@ -12,7 +12,10 @@
 
 namespace mfvm {
 
-// I32: the shell has fewer than 2^31 cells (32-bit index arithmetic).
+// I32: the shell has fewer than 2^31 cells (32-bit index arithmetic).  One
+// thread per ghost cell and field (the ensemble's fill, over many small
+// shells, takes one thread per cell for all fields instead; a single grid's
+// shell measured 4% faster with the fields spread over threads).
 template <class R, bool I32>
 __global__ void bc_kernel(BCArgs a, const R* __restrict__ src, R* __restrict__ dst, long long nshell) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;

@@ -36,7 +36,7 @@ struct EnsMem {
   int ntx, nty, ntz;        // stage tiles per axis
   int cnx, cny, cnz, ckz;   // closure tiles (64 x 4 x ckz)
   long long nshell;         // ghost cells per field
-  long long bc_off;         // first ghost-fill thread of this member
+  long long bc_off;         // first ghost-fill thread (ghost cell) of this member
   long long ntiles;         // stage tiles
   void* buf[3];             // stage states: U (hi), S1 (Sa), S2 (Sb)
   void* C;
@@ -63,11 +63,11 @@ MFVM_DEV long long ens_item(long long v, long long n, bool remap) {
   return (long long)(x * q + (x < r ? x : r) + ((unsigned long long)v >> 3));
 }
 
-// One thread per ghost cell and field of every member.  I32: fewer than 2^31
-// such threads in all (32-bit index arithmetic).  The member is found once
-// per wave, by a binary search on the wave's first thread; a wave that
-// straddles members takes them one after another.  The member index stays
-// wave-uniform, so every read of the member table is a scalar load.
+// One thread per ghost cell of every member, for all five fields.  I32: fewer
+// than 2^31 ghost cells in all (32-bit index arithmetic).  The member is
+// found once per wave, by a binary search on the wave's first thread; a wave
+// that straddles members takes them one after another.  The member index
+// stays wave-uniform, so every read of the member table is a scalar load.
 template <class R, bool I32>
 __global__ __launch_bounds__(kEnsBlock) void ens_bc_kernel(const EnsMem* __restrict__ mem, int nm, int src_sel,
                                                            int dst_sel, long long total) {
@@ -84,19 +84,15 @@ __global__ __launch_bounds__(kEnsBlock) void ens_bc_kernel(const EnsMem* __restr
     const EnsMem& e = mem[mb];
     const long long end = mb + 1 < nm ? mem[mb + 1].bc_off : total;
     if (t >= e.bc_off && t < end) {
-      int var, i, j, k;
-      if constexpr (I32) {
-        const unsigned u = (unsigned)(t - e.bc_off), ns = (unsigned)e.nshell, v = u / ns;
-        var = (int)v;
-        shell_cell32(e.lay, u - v * ns, i, j, k);
-      } else {
-        const long long u = t - e.bc_off;
-        var = (int)(u / e.nshell);
-        shell_cell(e.lay, u % e.nshell, i, j, k);
-      }
-      const R* src = static_cast<const R*>(e.buf[src_sel]) + var * e.lay.fs;
+      int i, j, k;
+      if constexpr (I32) shell_cell32(e.lay, (unsigned)(t - e.bc_off), i, j, k);
+      else shell_cell(e.lay, t - e.bc_off, i, j, k);
+      const R* src = static_cast<const R*>(e.buf[src_sel]);
       R* dst = static_cast<R*>(e.buf[dst_sel]);
-      dst[var * e.lay.fs + e.lay.at(i, j, k)] = shell_value<R>(e.bca, var, src, i, j, k);
+      const long long o = e.lay.at(i, j, k);
+#pragma unroll
+      for (int var = 0; var < 5; ++var)
+        dst[var * e.lay.fs + o] = shell_value<R>(e.bca, var, src + var * e.lay.fs, i, j, k);
     }
     if (t0 + 63 < end || mb + 1 >= nm || end >= total) break;
     ++mb;
@@ -645,7 +641,7 @@ hipError_t ens_advance(const mfvm_ens_member* m, int n, const mfvm_ns_params* p,
     e.cnz = ct[2];
     e.nshell = shell_count(mm.grid.n[0], mm.grid.n[1], mm.grid.n[2]);
     e.bc_off = bc_off;
-    bc_off += 5 * e.nshell;
+    bc_off += e.nshell;
     e.ntiles = (long long)nt[0] * nt[1] * nt[2];
     e.buf[0] = mm.U;
     e.buf[1] = mm.S1;
