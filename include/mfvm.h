@@ -313,8 +313,10 @@ int mfvm_ns_comp_advance(const mfvm_grid* g, const mfvm_ns_params* p, const mfvm
  * (src/solver.py:88-151); each member here advances exactly as
  * mfvm_ns_advance / mfvm_ns_comp_advance would advance it alone (same
  * arithmetic, its own CFL dt every step), while every phase of a stage (ghost
- * fill, closure, fused stage) is one launch over the tiles of all members,
- * and the steps can be replayed from captured hipGraphs. */
+ * fill, closure, the fused stage or the split x1/x2 and x3 passes) is one
+ * launch over a work list of all members (their tiles; for the split x1/x2
+ * pass, the row march's wave tasks), and the steps can be replayed from
+ * captured hipGraphs. */
 
 #define MFVM_ENS_ORDER_MEMBER 0      /* work list: member by member */
 #define MFVM_ENS_ORDER_COST 1        /* costliest tiles first (x3 planes per tile, then size) */
@@ -323,8 +325,9 @@ int mfvm_ns_comp_advance(const mfvm_grid* g, const mfvm_ns_params* p, const mfvm
 typedef struct mfvm_ens_member {
   mfvm_grid grid;               /* sizes and metric factors of this member */
   int32_t bc[5][6];             /* ghost rule per field and side */
-  int32_t tile[3];              /* stage tile x1, x2 threads (product 256) and x3 planes;
-                                   0 = 64, 4, min(nx3, 32) */
+  int32_t tile[3];              /* stage tile x1, x2 threads (product 256) and x3 planes
+                                   (fused stage and split x3 pass; the split x1/x2 pass is
+                                   the row march); 0 = 64, 4, min(nx3, 32) */
   void* U;                      /* ghosted state, advanced in place (compensated: hi, f32) */
   void* S1;                     /* ghosted stage states (compensated: Sa, Sb) */
   void* S2;
@@ -342,7 +345,8 @@ typedef struct mfvm_ens_member {
 typedef struct mfvm_ens_opts {
   int32_t order;                /* MFVM_ENS_ORDER_* */
   int32_t persist;              /* 0: one workgroup per tile; k > 0: k workgroups per CU,
-                                   each looping over the work list */
+                                   each looping over the tile work list (the row-march
+                                   x1/x2 pass always takes one wave per task) */
   int32_t graph;                /* > 0: steps per captured hipGraph (stream must not be the
                                    null stream); 0: direct launches */
   int32_t remap;                /* XCD-aware work order */
