@@ -1,8 +1,8 @@
 """Navier-Stokes HIP path vs the reference golden vectors and the oracle.
 
 Bar: bit-exact (every value ==, NaN == NaN) for the float64 path; the
-north-star tolerance (1e-12 relative L-inf after one step, 1e-10 after 200)
-is asserted as well where a long run is compared.
+float64 tolerance of the workload (1e-12 relative L-inf after one step,
+1e-10 after 200) is asserted as well where a long run is compared.
 """
 
 import os
