@@ -36,7 +36,10 @@ struct HeatArgs {
 // plane in registers (one new U of the column per plane, also read one plane
 // ahead).  Every metric is (a*b)*c over 1D factors: the plane-invariant a*b
 // products of the positions a thread uses are formed once, before the march,
-// and the x3 factors c of the chunk's planes are staged in LDS.
+// and the x3 factors c of the chunk's planes are staged in LDS.  The phi
+// values are double-buffered as well, so that the march runs one workgroup
+// barrier per plane: phi of a plane is formed in the phase that computes the
+// cells of the plane before it.
 // HW: half-width of the phi stencil of a face pair (1 without slopes, 2 with);
 // H = HW + 1: T ring.
 constexpr int HT_BX = 64, HT_BY = 4, HT_KZ = 16;  // tile, most planes per workgroup
@@ -51,12 +54,13 @@ template <int LIM> struct HeatTile {
   // one chunk (at most HT_KZ) and the HW + 1 (below) / HW - 1 (above) more
   // that phi_3 and the x3 faces reach
   static constexpr int NCP = HT_KZ + 2 * HW, NCT = 7 * NCP;
-  // LDS bytes: the factor table (double), then, in R, the two T buffers,
-  // P1 and HT_BX spare elements (the lanes of the wave that forms the phi_1
-  // edge positions beyond those positions write there), and P2
-  static constexpr size_t lds_bytes(size_t rs) {
-    return NCT * sizeof(double) + (2 * NTP + P1X * HT_BY + HT_BX + HT_BX * P2Y) * rs;
-  }
+  // one phi buffer: P1 and HT_BX spare elements (the lanes of the wave that
+  // forms the phi_1 edge positions beyond those positions write there), then
+  // P2
+  static constexpr int PB = P1X * HT_BY + HT_BX + HT_BX * P2Y;
+  // LDS bytes: the factor table (double), then, in R, the two T buffers and
+  // the two phi buffers
+  static constexpr size_t lds_bytes(size_t rs) { return NCT * sizeof(double) + (2 * NTP + 2 * PB) * rs; }
 };
 
 // The three factor rows of metric m: a (x1), b (x2), c (x3).
@@ -114,11 +118,12 @@ template <int LIM, class R, bool FAST>
 __global__ __launch_bounds__(HT_BX * HT_BY) void heat_march_kernel(HeatArgs a, int kz, int remap) {
   using TL = HeatTile<LIM>;
   constexpr int HW = TL::HW, H = TL::H, TX = TL::TX, NT = TL::NT, NTP = TL::NTP, SLOTS = TL::SLOTS;
-  constexpr int P1X = TL::P1X, NTH = HT_BX * HT_BY, NCP = TL::NCP;
+  constexpr int P1X = TL::P1X, NTH = HT_BX * HT_BY, NCP = TL::NCP, PB = TL::PB;
   constexpr bool SL = LIM != MFVM_LIM_NONE;
   extern __shared__ double smem_raw[];
   double* ct = smem_raw;                       // [7][NCP] x3 factors, from c-index kb-HW-1
   R* Tb = reinterpret_cast<R*>(ct + TL::NCT);  // [2][NTP]: [TY][TX], origin (i0-H, j0-H)
+  // two phi buffers, each P1 then P2 (PB elements)
   R* P1 = Tb + 2 * NTP;                        // [HT_BY][P1X], origin (i0-HW, j0), then HT_BX spare
   R* P2 = P1 + P1X * HT_BY + HT_BX;            // [P2Y][HT_BX], origin (i0, j0-HW)
   const Geo& g = a.geo;
@@ -263,24 +268,23 @@ __global__ __launch_bounds__(HT_BX * HT_BY) void heat_march_kernel(HeatArgs a, i
   tile_store(Tb);
   if (kb + 1 < ke) tile_load(kb + 1);
 
-  for (int m = kb; m < ke; ++m) {
-    const int cur = (m - kb) & 1;
-    const R* T = Tb + cur * NTP;
-    __syncthreads();  // Tb[cur] written; the previous plane's reads of P1 / P2 done
-    // phi_1 / phi_2 at every position this tile's faces use
-    {
-      const bool pw1 = g.lpow2[0] != 0, pw2 = g.lpow2[1] != 0;
-      const double c1 = cf(C_L1, m - 1), c2 = cf(C_L2, m - 1);
-      phi_task(p1o, T, 1, pw1, g.linv[0], c1, P1);
-      phi_task(p2o, T, TX, pw2, g.linv[1], c2, P2);
-      if (e1) phi_task(p1e, T, 1, pw1, g.linv[0], c1, P1);
-      if (e2) phi_task(p2e, T, TX, pw2, g.linv[1], c2, P2);
-    }
-    __syncthreads();  // P1 / P2 written; every read of Tb[cur ^ 1] (two planes back) done
-    if (m + 1 < ke) {
-      tile_store(Tb + (cur ^ 1) * NTP);  // plane m+1
-      if (m + 2 < ke) tile_load(m + 2);
-    }
+  // phi_1 / phi_2 at every position this tile's faces use, of plane m from
+  // the T buffer bf into the phi buffers bf
+  auto phis = [&](int m, int bf) {
+    const R* T = Tb + bf * NTP;
+    R* Q1 = P1 + bf * PB;
+    R* Q2 = P2 + bf * PB;
+    const bool pw1 = g.lpow2[0] != 0, pw2 = g.lpow2[1] != 0;
+    const double c1 = cf(C_L1, m - 1), c2 = cf(C_L2, m - 1);
+    phi_task(p1o, T, 1, pw1, g.linv[0], c1, Q1);
+    phi_task(p2o, T, TX, pw2, g.linv[1], c2, Q2);
+    if (e1) phi_task(p1e, T, 1, pw1, g.linv[0], c1, Q1);
+    if (e2) phi_task(p2e, T, TX, pw2, g.linv[1], c2, Q2);
+  };
+  // the own cell of plane m from the phi buffers bf, and its update
+  auto faces = [&](int m, int bf) {
+    const R* Q1 = P1 + bf * PB;
+    const R* Q2 = P2 + bf * PB;
     // x3: phi_3 of plane m+HW from the column; the window shifts by one
 #pragma unroll
     for (int t = 0; t < 2 * HW; ++t) ph3[t] = ph3[t + 1];
@@ -296,7 +300,7 @@ __global__ __launch_bounds__(HT_BX * HT_BY) void heat_march_kernel(HeatArgs a, i
     R ph[2 * HW + 1], len[3];
     R l1, h1, l2, h2, l3, h3;
 #pragma unroll
-    for (int t = -HW; t <= HW; ++t) ph[t + HW] = P1[ty * P1X + tx + HW + t];
+    for (int t = -HW; t <= HW; ++t) ph[t + HW] = Q1[ty * P1X + tx + HW + t];
     if constexpr (SL) {
 #pragma unroll
       for (int t = 0; t < 3; ++t) len[t] = g.lpow2[0] ? R(1.0 / g.linv[0]) : R(l1ab[t] * cf(C_L1, m - 1));
@@ -305,7 +309,7 @@ __global__ __launch_bounds__(HT_BX * HT_BY) void heat_march_kernel(HeatArgs a, i
     l1 = l1 * R(s1lo * cf(C_S1, m - 1));
     h1 = h1 * R(s1hi * cf(C_S1, m - 1));
 #pragma unroll
-    for (int t = -HW; t <= HW; ++t) ph[t + HW] = P2[(ty + HW + t) * HT_BX + tx];
+    for (int t = -HW; t <= HW; ++t) ph[t + HW] = Q2[(ty + HW + t) * HT_BX + tx];
     if constexpr (SL) {
 #pragma unroll
       for (int t = 0; t < 3; ++t) len[t] = g.lpow2[1] ? R(1.0 / g.linv[1]) : R(l2ab[t] * cf(C_L2, m - 1));
@@ -335,7 +339,31 @@ __global__ __launch_bounds__(HT_BX * HT_BY) void heat_march_kernel(HeatArgs a, i
       kr = kr + (a.qf ? static_cast<const R*>(a.qf)[a.il.at(i, j, m)] : R(a.q));
       stage_store<R>(a.io, 1, a.lay.fs, a.il.fs, a.lay.at(i, j, m), a.il.at(i, j, m), &u, &kr, nullptr);
     }
+  };
+
+  // One workgroup barrier per plane: phase m forms phi of plane m (T buffer
+  // and phi buffers (m - kb) & 1), stores the tile of plane m+1 into the
+  // other T buffer (whose plane m-1 the previous phase read) and computes the
+  // cells of plane m-1 from the other phi buffers (which the previous phase
+  // wrote; the phase before read them last).
+  __syncthreads();  // Tb[0] (plane kb) written
+  phis(kb, 0);
+  if (kb + 1 < ke) {
+    tile_store(Tb + NTP);
+    if (kb + 2 < ke) tile_load(kb + 2);
   }
+  __syncthreads();
+  for (int m = kb + 1; m < ke; ++m) {
+    const int cur = (m - kb) & 1;
+    phis(m, cur);
+    if (m + 1 < ke) {
+      tile_store(Tb + (cur ^ 1) * NTP);  // plane m+1
+      if (m + 2 < ke) tile_load(m + 2);
+    }
+    faces(m - 1, cur ^ 1);
+    __syncthreads();
+  }
+  faces(ke - 1, (ke - 1 - kb) & 1);
 }
 
 // dt = cfl / (0.0 + (2*nu)*smax), nu = k/(rho cp)   (heat.py:338-349)
