@@ -236,14 +236,18 @@ constexpr int MX_BX = 64, MX_BY = 4;
 
 // LDS of the in-plane exchange: RX[6][BY][BX+1] lower-face x1 states (the
 // last column: the cell beyond the tile), LX[6][BY] upper-face x1 states of
-// the cells before the tile, RY[6][BY+1][BX] / LY[6][BX] likewise for x2.
-// The face fluxes reuse the RX / RY space once every state is read.
+// the cells before the tile, RY[6][BY+1][BX] / LY[6][BX] likewise for x2,
+// then the face fluxes FX[6][BY][BX+1] and FY[6][BY+1][BX] in space of their
+// own, so that a plane takes two workgroup barriers (states written, fluxes
+// written): the next plane writes its states once every thread is past the
+// second barrier, after its last state read, and its fluxes once every
+// thread is past its first barrier, after its last flux read.
 constexpr int MX_NRX = MX_BY * (MX_BX + 1), MX_NRY = (MX_BY + 1) * MX_BX;
-constexpr size_t mx_lds_elems() { return 6 * (MX_NRX + MX_BY + MX_NRY + MX_BX); }
+constexpr size_t mx_lds_elems() { return 6 * (MX_NRX + MX_BY + MX_NRY + MX_BX) + 6 * (MX_NRX + MX_NRY); }
 
 // In-plane flux differences of plane m of the tile:
 // acc = ((F1(i+1/2) - F1(i-1/2)) + F2(j+1/2)) - F2(j-1/2).  Every thread of
-// the workgroup calls it (three barriers); cu: the own cell's values.
+// the workgroup calls it (two barriers); cu: the own cell's values.
 template <int LIM, int RS, class R>
 MFVM_DEV void mx_inplane(const MXArgs& a, const R* __restrict__ U, R* sm, int i0, int j0, int m, const R cu[6],
                          R acc[6]) {
@@ -253,14 +257,13 @@ MFVM_DEV void mx_inplane(const MXArgs& a, const R* __restrict__ U, R* sm, int i0
   R* LX = RX + 6 * nrx;
   R* RY = LX + 6 * BY;
   R* LY = RY + 6 * nry;
-  R* FX = RX;
-  R* FY = RY;
+  R* FX = LY + 6 * BX;
+  R* FY = FX + 6 * nrx;
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * BX + tx;
   const int n1 = a.lay.n1, n2 = a.lay.n2;
   const long long fs = a.lay.fs;
   const int i = i0 + tx, j = j0 + ty;
   const bool own = i <= n1 && j <= n2;
-  __syncthreads();  // the previous plane's flux reads are done
   R hx[6], hy[6];
   if (j <= n2 && i <= n1 + 1) {
     R lo[6];
@@ -381,7 +384,6 @@ MFVM_DEV void mx_inplane(const MXArgs& a, const R* __restrict__ U, R* sm, int i0
       }
     }
   }
-  __syncthreads();  // every state is read: the fluxes may overwrite them
   if (own) {
 #pragma unroll
     for (int q = 0; q < 6; ++q) {

@@ -670,17 +670,19 @@ MFVM_DEV void inplane_div(const StageArgs& a, const S& src, R* sm, int BX, int B
   const int wx = BX + 1;
   const int nrx = BY * wx, nry = (BY + 1) * BX;
   // LDS: RX[9][BY][BX+1] lower-face states of the x1 faces, LX[9][BY]
-  // upper-face states of the tile-edge cells, RY/LY likewise for x2.  The
-  // face fluxes FX[5][BY][BX+1] / FY[5][BY+1][BX] reuse the RX / RY space
-  // once every state has been read (one more barrier, a third less LDS).
+  // upper-face states of the tile-edge cells, RY/LY likewise for x2, then the
+  // face fluxes FX[5][BY][BX+1] / FY[5][BY+1][BX] in space of their own.  A
+  // plane takes two barriers (states written, fluxes written): the next
+  // plane writes its states once every thread is past the second one, after
+  // its last state read, and its fluxes once every thread is past the first
+  // one, after its last flux read.
   R* RX = sm;
   R* LX = RX + NQ * nrx;
   R* RY = LX + NQ * BY;
   R* LY = RY + NQ * nry;
-  R* FX = RX;
-  R* FY = RY;
+  R* FX = LY + NQ * BX;
+  R* FY = FX + 5 * nrx;
   const int NT = BX * BY;
-  __syncthreads();  // the previous plane's reads of FX / FY are done
   R hx[NQ], hy[NQ];
   if (j <= n2 && i <= n1 + 1) {
     R lo[NQ];
@@ -808,7 +810,6 @@ MFVM_DEV void inplane_div(const StageArgs& a, const S& src, R* sm, int BX, int B
       }
     }
   }
-  __syncthreads();  // every state read: the flux arrays may overwrite them
   if (own) {
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
@@ -1823,7 +1824,7 @@ inline Tile ns_tile(const mfvm_grid* g, const mfvm_ns_params* p) {
 // LDS bytes of the stage body for a bx x by tile.
 inline size_t stage_lds_bytes(int bx, int by, size_t rsize) {
   const size_t nrx = (size_t)by * (bx + 1), nry = (size_t)(by + 1) * bx;
-  size_t lds = (NQ * nrx + NQ * by + NQ * nry + NQ * bx) * rsize;
+  size_t lds = (NQ * nrx + NQ * by + NQ * nry + NQ * bx + 5 * nrx + 5 * nry) * rsize;
   const size_t red = 2 * sizeof(double) * ((bx * by + 63) / 64);
   return lds < red ? red : lds;
 }
