@@ -15,6 +15,11 @@ template <class R> hipError_t launch_cfl(const mfvm_grid*, const mfvm_ns_params*
 hipError_t launch_dt_finalize(const mfvm_grid*, const mfvm_ns_params*, const double*, double, double*, hipStream_t);
 long long ns_nblocks(const mfvm_grid*, const mfvm_ns_params*);
 template <class R> hipError_t launch_bcs(const mfvm_grid*, int, const int32_t*, const void*, void*, hipStream_t);
+hipError_t launch_select_cfl_partials(const mfvm_grid* g, const mfvm_ns_params* p, const void* T, void* U,
+                                      const int* flag, int* count, double* partials, hipStream_t s);
+template <class R>
+bool launch_select_bcs(const mfvm_grid*, int, const int32_t*, const void*, void*, const int*, int*, hipStream_t,
+                       hipError_t*);
 template <class R> hipError_t launch_copy_interior(const mfvm_grid*, int, const void*, void*, hipStream_t);
 template <class R> hipError_t heat_launch_stage(const mfvm_grid*, const mfvm_heat_params*, const void*, const StageIO&, hipStream_t);
 hipError_t heat_launch_dt(const mfvm_grid*, const mfvm_heat_params*, double, double*, hipStream_t);
@@ -730,8 +735,15 @@ static int closure_step(const mfvm_grid* g, const mfvm_ns_params* p, const mfvm_
   const int32_t* bc = &p->bc[0][0];
   // (the stage flags start at 0: the previous step's dt append, or the set-up, cleared them)
   hipError_t e = hipSuccess;
+  // (pending: the selection of stage q-1 is still to be made, fused with
+  // this stage's ghost fill; the last stage's, with the CFL partials)
+  bool pending = false;
   for (int q = 0; q < 3 && e == hipSuccess; ++q) {
-    e = launch_bcs<double>(g, 5, bc, seq[q].in, seq[q].in, s);
+    if (!pending || !launch_select_bcs<double>(g, 5, bc, cw->T, seq[q].in, cw->flag + q - 1, cw->counts + q - 1, s, &e)) {
+      if (pending) e = closure_select<double>(g, cw->T, seq[q].in, cw->flag + q - 1, cw->counts + q - 1, s);
+      if (e == hipSuccess) e = launch_bcs<double>(g, 5, bc, seq[q].in, seq[q].in, s);
+    }
+    pending = false;
     if (e == hipSuccess) e = launch_closure<double>(g, p, seq[q].in, w->C, s);
     mfvm_stage st = {};
     st.U = seq[q].in;
@@ -766,9 +778,11 @@ static int closure_step(const mfvm_grid* g, const mfvm_ns_params* p, const mfvm_
         e = closure_apply<double>(g, p->gamma, seq[q].out, cw->y, net->out_scale, seq[q].coef, cw->dt_cur, cw->T,
                                   cw->flag + q, s);
     }
-    if (e == hipSuccess) e = closure_select<double>(g, cw->T, seq[q].out, cw->flag + q, cw->counts + q, s);
+    pending = true;
   }
-  if (e == hipSuccess) e = launch_cfl_partials<double>(g, p, w->U, nullptr, w->partials, s);
+  if (e == hipSuccess)
+    e = pending ? launch_select_cfl_partials(g, p, cw->T, w->U, cw->flag + 2, cw->counts + 2, w->partials, s)
+                : launch_cfl_partials<double>(g, p, w->U, nullptr, w->partials, s);
   if (e == hipSuccess)
     e = launch_dt_append(g, p, w->partials, cfl, t_end, w->dts, cw->dt_cur, cw->t_cur, cw->step, cw->flag, 3, s);
   return hip_check(e, "closure_advance");

@@ -129,6 +129,39 @@ __global__ __launch_bounds__(256) void ns_cfl_kernel(Layout lay, Geo geo, NSPar 
   cfl_tile_body<UNIF, R>(lay, geo, p, U, lo, kz, t);
 }
 
+// A learned-closure step's last selection fused with the CFL partials that
+// follow it: unless *flag is set, the corrected state T replaces the
+// interior of U; the partials are those of the selected state (T, or U when
+// the flag is set, whose interior then stays and the stage is counted).  The
+// copy into U never meets a read of U: the partials then read T.
+template <bool UNIF, class R>
+__global__ __launch_bounds__(256) void ns_select_cfl_kernel(Layout lay, Geo geo, NSPar p, const R* __restrict__ T,
+                                                           R* U, const int* __restrict__ flag,
+                                                           int* __restrict__ count, double* partials, int kz) {
+  const bool keep = *flag != 0;
+  TileCtx t;
+  t.x = blockIdx.x;
+  t.y = blockIdx.y;
+  t.z = blockIdx.z;
+  t.bx = blockDim.x;
+  t.by = blockDim.y;
+  t.slot = partials + 2 * hw_block_id();
+  if (!keep) {
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    const int i = 1 + t.x * t.bx + tid % t.bx, j = 1 + t.y * t.by + tid / t.bx;
+    const int kb = 1 + t.z * kz, ke = min(kb + kz, lay.n3 + 1);
+    if (i <= lay.n1 && j <= lay.n2)
+      for (int k = kb; k < ke; ++k) {
+        const long long o = lay.at(i, j, k);
+#pragma unroll
+        for (int q = 0; q < 5; ++q) U[o + q * lay.fs] = T[o + q * lay.fs];
+      }
+  } else if (hw_block_id() == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
+    atomicAdd(count, 1);
+  }
+  cfl_tile_body<UNIF, R>(lay, geo, p, keep ? U : T, nullptr, kz, t);
+}
+
 __global__ void ns_dt_finalize_kernel(const double* partials, long long nblk, double cfl,
                                       double smax, double* dt_out) {
   const double dt = dt_reduce_block(partials, nblk, cfl, smax);
@@ -366,6 +399,27 @@ hipError_t launch_cfl_partials(const mfvm_grid* g, const mfvm_ns_params* p, cons
   else
     hipLaunchKernelGGL((ns_cfl_kernel<false, R>), grd, blk, red, s, lay, geo, q, static_cast<const R*>(U),
                        static_cast<const R*>(lo), partials, t.kz);
+  return hipGetLastError();
+}
+
+// ns_select_cfl_kernel: launch_cfl_partials of the selected state (float64).
+hipError_t launch_select_cfl_partials(const mfvm_grid* g, const mfvm_ns_params* p, const void* T, void* U,
+                                      const int* flag, int* count, double* partials, hipStream_t s) {
+  const Tile t = ns_tile(g, p);
+  const Layout lay = make_layout(g->n[0], g->n[1], g->n[2]);
+  const Geo geo = make_geo(g);
+  const NSPar q = make_ns_par(p);
+  const dim3 blk(t.bx, t.by, 1);
+  const dim3 grd = ns_grid(g, t);
+  const size_t red = 2 * sizeof(double) * ((t.bx * t.by + 63) / 64);
+  const double* tt = static_cast<const double*>(T);
+  double* u = static_cast<double*>(U);
+  if (g->uniform)
+    hipLaunchKernelGGL((ns_select_cfl_kernel<true, double>), grd, blk, red, s, lay, geo, q, tt, u, flag, count,
+                       partials, t.kz);
+  else
+    hipLaunchKernelGGL((ns_select_cfl_kernel<false, double>), grd, blk, red, s, lay, geo, q, tt, u, flag, count,
+                       partials, t.kz);
   return hipGetLastError();
 }
 
