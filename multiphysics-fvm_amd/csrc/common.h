@@ -169,8 +169,12 @@ struct Geo {
 };
 
 // i, j, k are 0-based interior (or face) indices of the metric's own array.
+// The factor rows are read-only while any kernel runs, so they are read
+// through the constant address space: where an index is wave-uniform the
+// compiler may then take a scalar load (its value in an SGPR).
+typedef const __attribute__((address_space(4))) double ConstF64;
 template <class R> MFVM_DEV R met(const Geo& g, int m, int i, int j, int k) {
-  const double* f = g.fac + (long long)(3 * m) * g.fstride;
+  ConstF64* f = (ConstF64*)(g.fac + (long long)(3 * m) * g.fstride);
   return R((f[i] * f[g.fstride + j]) * f[2 * g.fstride + k]);
 }
 

@@ -42,36 +42,62 @@ MFVM_DEV R mx_E(const MXPar& p, R d) { return p.ep_pow2 ? d * R(p.inv_ep) : d / 
 // ---------------------------------------------------------------------------
 // Residuals
 // ---------------------------------------------------------------------------
+// div B and div D of interior cell (i,j,k) (maxwell.py:491-508): centred
+// differences over 2 L_d, plus on curvilinear grids the scale-factor terms
+// (F_d (dh_d' + dh_d'') / h_d).  The two fields are formed together: the
+// metrics once for both, and each denominator's two quotients share one
+// reciprocal refinement.
 template <class R>
-MFVM_DEV R mx_div(const Layout& lay, const Geo& g, const R* F, int i, int j, int k) {
+MFVM_DEV void mx_div2(const Layout& lay, const Geo& g, const R* U, int i, int j, int k, R& dB, R& dD) {
   const long long o = lay.at(i, j, k), fs = lay.fs;
   const long long st[3] = {1, lay.sy, lay.sz};
-  R gr[3];
+  R gb[3], gd[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const R d = F[o + a * fs + st[a]] - F[o + a * fs - st[a]];
-    gr[a] = g.lpow2[a] ? d * R(0.5 * g.linv[a]) : d / (R(2) * met<R>(g, MFVM_MET_L1 + a, i - 1, j - 1, k - 1));
+    const R nb = U[o + a * fs + st[a]] - U[o + a * fs - st[a]];
+    const R nd = U[o + (3 + a) * fs + st[a]] - U[o + (3 + a) * fs - st[a]];
+    if (g.lpow2[a]) {
+      gb[a] = nb * R(0.5 * g.linv[a]);
+      gd[a] = nd * R(0.5 * g.linv[a]);
+    } else {
+      const R n[2] = {nb, nd};
+      R q[2];
+      ieee_div_by<2, R>(n, R(2) * met<R>(g, MFVM_MET_L1 + a, i - 1, j - 1, k - 1), q);
+      gb[a] = q[0];
+      gd[a] = q[1];
+    }
   }
-  R div = (gr[0] + gr[1]) + gr[2];
+  dB = (gb[0] + gb[1]) + gb[2];
+  dD = (gd[0] + gd[1]) + gd[2];
   if (g.curv) {
     const int a = i - 1, b = j - 1, e = k - 1;
-    const R c1 = F[o], c2 = F[o + fs], c3 = F[o + 2 * fs];
-    div = ((div + (c1 * (met<R>(g, MFVM_MET_D1H2, a, b, e) + met<R>(g, MFVM_MET_D1H3, a, b, e))) /
-                      met<R>(g, MFVM_MET_H1, a, b, e)) +
-           (c2 * (met<R>(g, MFVM_MET_D2H1, a, b, e) + met<R>(g, MFVM_MET_D2H3, a, b, e))) /
-               met<R>(g, MFVM_MET_H2, a, b, e)) +
-          (c3 * (met<R>(g, MFVM_MET_D3H1, a, b, e) + met<R>(g, MFVM_MET_D3H2, a, b, e))) /
-              met<R>(g, MFVM_MET_H3, a, b, e);
+    const R m1 = met<R>(g, MFVM_MET_D1H2, a, b, e) + met<R>(g, MFVM_MET_D1H3, a, b, e);
+    const R m2 = met<R>(g, MFVM_MET_D2H1, a, b, e) + met<R>(g, MFVM_MET_D2H3, a, b, e);
+    const R m3 = met<R>(g, MFVM_MET_D3H1, a, b, e) + met<R>(g, MFVM_MET_D3H2, a, b, e);
+    const R h[3] = {met<R>(g, MFVM_MET_H1, a, b, e), met<R>(g, MFVM_MET_H2, a, b, e), met<R>(g, MFVM_MET_H3, a, b, e)};
+    const R mm[3] = {m1, m2, m3};
+    R tb[3], td[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const R n[2] = {U[o + c * fs] * mm[c], U[o + (3 + c) * fs] * mm[c]};
+      R q[2];
+      ieee_div_by<2, R>(n, h[c], q);
+      tb[c] = q[0];
+      td[c] = q[1];
+    }
+    dB = ((dB + tb[0]) + tb[1]) + tb[2];
+    dD = ((dD + td[0]) + td[1]) + td[2];
   }
-  return div;
 }
 
 // rhof: per-cell charge density (interior layout) or NULL: p.rho everywhere.
+// A workgroup row is one wave (blockDim.x = 64), so j is wave-uniform and the
+// x2 and x3 metric factors are scalar loads.
 template <class R>
 __global__ void maxwell_res_kernel(Layout lay, Geo geo, MXPar p, const R* __restrict__ U, R* __restrict__ Rr,
                                    const R* __restrict__ rhof) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int j = blockIdx.y * blockDim.y + __builtin_amdgcn_readfirstlane(threadIdx.y);
   const int k = blockIdx.z;
   const int n1 = lay.n1, n2 = lay.n2, n3 = lay.n3;
   if (i > n1 + 1 || j > n2 + 1) return;
@@ -81,9 +107,10 @@ __global__ void maxwell_res_kernel(Layout lay, Geo geo, MXPar p, const R* __rest
   const int ii = i == 0 ? 1 : (i == n1 + 1 ? n1 : i);
   const int jj = j == 0 ? 1 : (j == n2 + 1 ? n2 : j);
   const int kk = k == 0 ? 1 : (k == n3 + 1 ? n3 : k);
-  const R rB = mx_div<R>(lay, geo, U, ii, jj, kk);
+  R rB, dD;
+  mx_div2<R>(lay, geo, U, ii, jj, kk, rB, dD);
   const long long ic = (long long)(kk - 1) * lay.n2 * lay.n1 + (long long)(jj - 1) * lay.n1 + (ii - 1);
-  const R rD = mx_div<R>(lay, geo, U + 3 * lay.fs, ii, jj, kk) - (rhof ? rhof[ic] : R(p.rho));
+  const R rD = dD - (rhof ? rhof[ic] : R(p.rho));
   const long long o = lay.at(i, j, k);
   Rr[o] = rB;
   Rr[o + lay.fs] = rD;
@@ -259,7 +286,7 @@ MFVM_DEV void mx_inplane(const MXArgs& a, const R* __restrict__ U, R* sm, int i0
   R* LY = RY + 6 * nry;
   R* FX = LY + 6 * BX;
   R* FY = FX + 6 * nrx;
-  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * BX + tx;
+  const int tx = threadIdx.x, ty = __builtin_amdgcn_readfirstlane(threadIdx.y), tid = ty * BX + tx;
   const int n1 = a.lay.n1, n2 = a.lay.n2;
   const long long fs = a.lay.fs;
   const int i = i0 + tx, j = j0 + ty;
@@ -423,7 +450,7 @@ MFVM_DEV void mx_stage_body(const MXArgs& a, int kz, int remap) {
   const int n1 = a.lay.n1, n2 = a.lay.n2, n3 = a.lay.n3;
   const long long fs = a.lay.fs;
   const int i0 = 1 + tl.x * MX_BX, j0 = 1 + tl.y * MX_BY;
-  const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
+  const int i = i0 + threadIdx.x, j = j0 + __builtin_amdgcn_readfirstlane(threadIdx.y);  // a row is one wave
   const int kb = 1 + tl.z * kz, ke = min(kb + kz, n3 + 1);
   const bool own = i <= n1 && j <= n2;
   const R* __restrict__ U = static_cast<const R*>(a.U);
@@ -513,7 +540,7 @@ __global__ __launch_bounds__(MX_BX * MX_BY, 2) void maxwell_xy_kernel(MXArgs a, 
   R* sm = reinterpret_cast<R*>(smem_raw);
   const TileId tl = tile_id(remap != 0);
   const int i0 = 1 + tl.x * MX_BX, j0 = 1 + tl.y * MX_BY;
-  const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
+  const int i = i0 + threadIdx.x, j = j0 + __builtin_amdgcn_readfirstlane(threadIdx.y);  // a row is one wave
   const int kb = 1 + tl.z * kz, ke = min(kb + kz, a.lay.n3 + 1);
   const bool own = i <= a.lay.n1 && j <= a.lay.n2;
   const R* __restrict__ U = static_cast<const R*>(a.U);
