@@ -158,10 +158,13 @@ class Ensemble:
             out.append(V)
         return out
 
-    def autotune(self, cfl, steps=3, verbose=False):
+    def autotune(self, cfl, steps=3, verbose=False, repeats=2, min_gain=0.01):
         """Pick split or fused stages, tiles per size class, work order,
         persistence and XCD order by coordinate descent over timed runs of
-        ``steps`` steps on copies of the members.  Stores and returns the
+        ``steps`` steps on copies of the members.  Each candidate's time is
+        the least of ``repeats`` runs, and a candidate replaces the current
+        choice only when it is faster by more than ``min_gain`` (a fraction),
+        so that run-to-run noise does not decide.  Stores and returns the
         chosen options."""
         shadow = self._shadow()
         log = []
@@ -169,11 +172,13 @@ class Ensemble:
         def timed(opts):
             o = dict(opts, graph=0)
             self._run(shadow, o, cfl, 1)  # warm-up (and table sizing)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            self._run(shadow, o, cfl, steps)
-            torch.cuda.synchronize()
-            ms = (time.perf_counter() - t0) * 1e3 / steps
+            ms = float("inf")
+            for _ in range(repeats):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                self._run(shadow, o, cfl, steps)
+                torch.cuda.synchronize()
+                ms = min(ms, (time.perf_counter() - t0) * 1e3 / steps)
             log.append((dict(o, tiles=dict(o["tiles"])), ms))
             if verbose:
                 print(f"autotune split={o['split']} {o['order']:>10} persist={o['persist']} "
@@ -184,7 +189,7 @@ class Ensemble:
         best_ms = timed(best)
         trial = dict(best, split=1 - best["split"])
         ms = timed(trial)
-        if ms < best_ms:
+        if ms < best_ms * (1.0 - min_gain):
             best, best_ms = trial, ms
         classes = {}
         for U in self.states:
@@ -199,7 +204,7 @@ class Ensemble:
                 tiles[shape] = t
                 trial = dict(best, tiles=tiles)
                 ms = timed(trial)
-                if ms < best_ms:
+                if ms < best_ms * (1.0 - min_gain):
                     best, best_ms = trial, ms
         for key, values in (("order", ("member", "interleave", "cost")), ("persist", (1, 2, 0)),
                             ("remap", (0, 1))):
@@ -208,7 +213,7 @@ class Ensemble:
                     continue
                 trial = dict(best, **{key: v})
                 ms = timed(trial)
-                if ms < best_ms:
+                if ms < best_ms * (1.0 - min_gain):
                     best, best_ms = trial, ms
         best["graph"] = DEFAULT_OPTIONS["graph"]
         self.options = best
