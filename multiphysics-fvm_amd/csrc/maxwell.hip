@@ -607,9 +607,11 @@ hipError_t maxwell_launch_res(const mfvm_grid* g, const mfvm_maxwell_params* q, 
 }
 
 // x3 planes per workgroup: the fused stage and the x3 pass of the split
-// stage (MX_KZ), the x1/x2 pass (MX_XY_KZ: it keeps nothing from one plane
-// to the next, so short chunks only add workgroups).
-constexpr int MX_KZ = 16, MX_XY_KZ = 8;
+// stage (MX_KZ: 32 measures 0.7% faster than 16 and 1.6% faster than 8 at
+// 128x128x256, the march's set-up spread over more planes), the x1/x2 pass
+// (MX_XY_KZ: it keeps nothing from one plane to the next, so short chunks
+// only add workgroups).
+constexpr int MX_KZ = 32, MX_XY_KZ = 8;
 
 template <int LIM, int RS, class R>
 static hipError_t mx_launch_t(const MXArgs& a, hipStream_t s) {
