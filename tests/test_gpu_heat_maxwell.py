@@ -253,6 +253,18 @@ def test_heat_march_chunks_match_oracle(rec, lim, rho_cp, z_max):
     takes the specialised power-of-two Euler form with a power-of-two x1 and
     x3 length (as the cylindrical heat benchmark does); rho*cp = 1.5 the
     general form."""
+    _heat_march_case(rec, lim, rho_cp, z_max, 72)
+
+
+@pytest.mark.parametrize("rho_cp", [1.0, 1.5])
+def test_heat_march_odd_and_partial_chunks_match_oracle(rho_cp):
+    """256 x 64 x 50: three planes per workgroup (an odd count, so the march's
+    two phi and T buffers end on the other parity) and a last chunk of two:
+    the right-hand side and 3 Euler steps equal the oracle's bit for bit."""
+    _heat_march_case("piecewise_constant", "minmod", rho_cp, 50 / 128, 50)
+
+
+def _heat_march_case(rec, lim, rho_cp, z_max, n3):
     from oracle import grid as og
     from oracle import heat as oht
     from oracle import numerics as on
@@ -261,7 +273,7 @@ def test_heat_march_chunks_match_oracle(rec, lim, rho_cp, z_max):
     from mfvm.grid.geometry import build_grid
     from mfvm.numerics import time_integrators as ti
     from mfvm.physics import heat
-    n = (256, 64, 72)
+    n = (256, 64, n3)
     gc = grid_config(n, (0.0, 0.0, 0.0), (1.0, 2 * np.pi, z_max))
     params = {"mass_density": 1.0, "specific_heat_capacity": rho_cp, "thermal_conductivity": 1e-2,
               "volumetric_heat_source": 0.0}
